@@ -1,0 +1,132 @@
+/*
+ * ld.h — C-ABI of the MI355X forced-DNS + droplet step library (libld.so).
+ *
+ * Drop-in boundary for the hot path of the reference (Rajarshi-prime/
+ * lucky-droplets): forced-dns-sm.py (spectral flow, RK4, step epilogue) and
+ * particles.py (droplets).  The reference has no FFI of its own: its hot path
+ * is a set of Python functions on module-level numpy arrays.  Each entry
+ * point below replaces one of those functions; the comment names the
+ * reference function (file:line) and the argument meaning is kept:
+ * caller-owned buffers are written in place, as the reference mutates its
+ * preallocated arrays.
+ *
+ * Conventions
+ *  - All buffer arguments are DEVICE pointers (e.g. torch tensor.data_ptr()).
+ *  - Field precision follows the context: LD_F32 -> float / complex64,
+ *    LD_F64 -> double / complex128.  Droplet rows are always float64.
+ *  - Layouts are the reference's at one MPI rank (C order):
+ *      physical field  (N, N, N)          [x][y][z]
+ *      spectral field  (N, N, N/2+1)      [kx][ky][kz], complex
+ *      vector field    3 fields back to back, component-major
+ *      droplet rows    (Np, 7) float64    [x, y, z, vx, vy, vz, m/M0]
+ *  - Forward transforms are unnormalised; inverse transforms carry 1/N^3.
+ *  - Every call is asynchronous on the context's stream (ld_set_stream).
+ *  - Return value: LD_OK or an LD_ERR_* code; ld_last_error() gives text.
+ *  - Contexts are independent; a context must not be used from two threads
+ *    at once.
+ */
+#ifndef LD_H
+#define LD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LD_OK 0
+#define LD_ERR_ARG 1        /* bad argument (size, null pointer, unsupported N) */
+#define LD_ERR_HIP 2        /* HIP runtime error */
+#define LD_ERR_BLOWUP 3     /* max Re(u_hat) > 100 N^3 (forced-dns-sm.py:618-620) */
+#define LD_ERR_CAPACITY 4   /* more droplets than the context was created for */
+
+#define LD_F32 0
+#define LD_F64 1
+
+/* Physical and numerical parameters: the reference's module constants. */
+typedef struct ld_params {
+  int32_t N;             /* grid points per side; power of two, 16..512       forced-dns-sm.py:28 */
+  int32_t precision;     /* LD_F32 or LD_F64 (fields and transforms) */
+  double L;              /* box side, 2*pi                                    forced-dns-sm.py:54 */
+  double nu;             /* viscosity                                         forced-dns-sm.py:83 */
+  int32_t lp;            /* viscosity power; only 1                           forced-dns-sm.py:78 */
+  double isexplicit;     /* 1.0: explicit viscosity                           forced-dns-sm.py:23 */
+  int32_t nshells;       /* number of forced shells (<= 8)                    forced-dns-sm.py:89 */
+  int32_t shell_no[8];   /* forced shells, each in [1, 8]                     forced-dns-sm.py:90 */
+  double f0;             /* power input per shell                             forced-dns-sm.py:97 */
+  double isforcing;      /* 1.0 forcing on                                    forced-dns-sm.py:19 */
+  double tps;            /* small-droplet relaxation time tau_p = sts*tf      forced-dns-sm.py:93 */
+  double g;              /* gravity in simulation units                       forced-dns-sm.py:94 */
+  double st_s;           /* small-droplet Stokes number (sts)                 particles.py:95 */
+  double tau_eta;        /* Kolmogorov time used by the droplets (tf)         particles.py:56 */
+  double factor;         /* m/M0 = St^1.5 * factor                            particles.py:74 */
+  double growthfactor;   /* 9 pi nu / (2 rho_p)                               particles.py:75 */
+  int64_t max_particles; /* droplet capacity of the context */
+} ld_params;
+
+typedef struct ld_ctx ld_ctx;
+
+/* Context: plans, twiddle tables and all workspace are allocated here, never
+ * inside a step.  Replaces the module-level setup of forced-dns-sm.py:49-194
+ * and MPI_particles.__init__/to_interp/to_exterp (particles.py:54-142). */
+int ld_create(const ld_params* params, ld_ctx** out);
+int ld_destroy(ld_ctx* ctx);
+int ld_set_stream(ld_ctx* ctx, void* hip_stream);
+/* Copies the calling thread's last error message (NUL-terminated). */
+int ld_last_error(char* buf, size_t len);
+/* Library version string, e.g. "ld 0.1 gfx950". */
+const char* ld_version(void);
+
+/* ---------------------------------------------------------------- flow */
+/* rfft_mpi(u, fu): 3-D r2c of nfields real fields        forced-dns-sm.py:201-206 */
+int ld_rfft(ld_ctx* ctx, const void* u, void* fu, int nfields);
+/* irfft_mpi(fu, u): 3-D c2r of nfields fields, 1/N^3     forced-dns-sm.py:208-213 */
+int ld_irfft(ld_ctx* ctx, const void* fu, void* u, int nfields);
+/* forcing(uk, fk): constant-power solenoidal forcing     forced-dns-sm.py:239-275
+ * ek_out (may be NULL): device double[N/2+1] shell energies of the forced shells. */
+int ld_forcing(ld_ctx* ctx, const void* uk, void* fk, double* ek_out);
+/* clip_zero(x): x clipped in place, out = x*oldmean/newmean  forced-dns-sm.py:278-285 */
+int ld_clip_zero(ld_ctx* ctx, void* x, void* out);
+/* full_RHS(t, uk, n, sump, coord, stb, ku, kn, fc, visc, forc)   forced-dns-sm.py:287-332
+ * coord: droplet stage rows (np, 7); kp: droplet RHS rows (np, 7); fc: deposit field.
+ * sump is passed through unchanged (rows keep their order on one device).
+ * u_out, v_out (may be NULL): the physical u and slow-manifold v (3 fields each). */
+int ld_full_rhs(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np,
+                void* ku, void* kn, double* kp, void* fc, double visc, double forc,
+                void* u_out, void* v_out);
+/* RK4(t, h, stb, uk, n, uknew, nnew) -> (sump, uknew, nnew)  forced-dns-sm.py:335-362
+ * n is clipped in place as the reference's first clip_zero does (:341).
+ * coord: droplet rows in (np, 7); coord_new: rows out (sump). */
+int ld_rk4(ld_ctx* ctx, double h, const void* uk, void* n, const double* coord, int64_t np,
+           void* uknew, void* nnew, double* coord_new);
+/* Loop body of evolve_and_save, nsteps times            forced-dns-sm.py:582-620
+ * (RK4, update_intrinsic, clip_zero, reality round trip, projection, blow-up
+ * check) on uk (3 spectral fields), n (physical) and droplet rows, in place.
+ * Returns LD_ERR_BLOWUP if any step's max Re(uk) exceeded 100 N^3. */
+int ld_step(ld_ctx* ctx, double h, void* uk, void* n, double* coord, int64_t np, int32_t nsteps);
+
+/* Context-resident state (the fast path: no layout conversion per step). */
+int ld_state_set(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np);
+int ld_state_get(ld_ctx* ctx, void* uk, void* n, double* coord);
+int ld_advance(ld_ctx* ctx, double h, int32_t nsteps);
+/* Last step's max Re(uk) (host double) and sticky blow-up flag. */
+int ld_status(ld_ctx* ctx, double* last_max, int32_t* blown_up);
+
+/* ------------------------------------------------------------ droplets */
+/* MPI_particles.interp_cosine(coord, u): 4x4x4 cosine-kernel gather of ncomp
+ * physical fields at the rows' positions -> out (np, ncomp)   particles.py:435-511 */
+int ld_interp_cosine(ld_ctx* ctx, const double* coord, int64_t np, const void* fields, int32_t ncomp,
+                     double* out);
+/* MPI_particles.exterp_cosine_scalar(coord, c): c = deposit of w / dx^3
+ * (c is zeroed first, particles.py:521)                        particles.py:514-577 */
+int ld_exterp_cosine_scalar(ld_ctx* ctx, const double* coord, int64_t np, const double* w, void* c);
+/* MPI_particles.pRHS(t, coord, u, us, c, fc, sump) -> (sump, rhs, fc)  particles.py:817-837
+ * Also returns interpmat (np, 7) and exterpmat (np).  n is used as given. */
+int ld_prhs(ld_ctx* ctx, const double* coord, int64_t np, const void* u, const void* us, const void* n,
+            void* fc, double* rhs, double* interpmat, double* exterpmat);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LD_H */

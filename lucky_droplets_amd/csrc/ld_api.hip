@@ -1,0 +1,207 @@
+// C-ABI entry points of libld.so (declared in include/ld.h).
+#include "ld_engine_base.hpp"
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+struct ld_ctx {
+  ld::EngineBase* eng = nullptr;
+  ld_params p{};
+};
+
+namespace ld {
+static thread_local char g_err[1024] = "";
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+}  // namespace ld
+
+using ld::set_error;
+
+#define LD_CHECK_CTX(c)                                 \
+  do {                                                  \
+    if (!(c) || !(c)->eng) {                            \
+      set_error("null or destroyed ld_ctx");            \
+      return LD_ERR_ARG;                                \
+    }                                                   \
+  } while (0)
+#define LD_NONNULL(ptr)                                 \
+  do {                                                  \
+    if (!(ptr)) {                                       \
+      set_error("%s: null pointer argument " #ptr, __func__); \
+      return LD_ERR_ARG;                                \
+    }                                                   \
+  } while (0)
+
+static ld::EngineBase* make_engine(const ld_params& p) {
+  using namespace ld;
+#define LD_CASE(TN, NN) \
+  if (p.N == NN) return make_engine_##TN##_##NN(p);
+  if (p.precision == LD_F32) {
+    LD_CASE(f32, 16) LD_CASE(f32, 32) LD_CASE(f32, 64) LD_CASE(f32, 128) LD_CASE(f32, 256) LD_CASE(f32, 512)
+  } else if (p.precision == LD_F64) {
+    LD_CASE(f64, 16) LD_CASE(f64, 32) LD_CASE(f64, 64) LD_CASE(f64, 128) LD_CASE(f64, 256) LD_CASE(f64, 512)
+  }
+#undef LD_CASE
+  return nullptr;
+}
+
+extern "C" {
+
+const char* ld_version(void) { return "ld 0.1 gfx950"; }
+
+int ld_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return LD_ERR_ARG;
+  strncpy(buf, ld::g_err, len - 1);
+  buf[len - 1] = '\0';
+  return LD_OK;
+}
+
+int ld_create(const ld_params* params, ld_ctx** out) {
+  LD_NONNULL(params);
+  LD_NONNULL(out);
+  *out = nullptr;
+  if (params->precision != LD_F32 && params->precision != LD_F64) {
+    set_error("precision must be LD_F32 or LD_F64");
+    return LD_ERR_ARG;
+  }
+  ld::EngineBase* e = make_engine(*params);
+  if (!e) {
+    set_error("unsupported grid size N=%d (power of two in [16, 512])", params->N);
+    return LD_ERR_ARG;
+  }
+  int rc = e->init();
+  if (rc != LD_OK) {
+    delete e;
+    return rc;
+  }
+  ld_ctx* c = new (std::nothrow) ld_ctx;
+  if (!c) {
+    delete e;
+    set_error("out of host memory");
+    return LD_ERR_ARG;
+  }
+  c->eng = e;
+  c->p = *params;
+  *out = c;
+  return LD_OK;
+}
+
+int ld_destroy(ld_ctx* ctx) {
+  if (!ctx) return LD_OK;
+  delete ctx->eng;
+  ctx->eng = nullptr;
+  delete ctx;
+  return LD_OK;
+}
+
+int ld_set_stream(ld_ctx* ctx, void* s) {
+  LD_CHECK_CTX(ctx);
+  ctx->eng->set_stream((hipStream_t)s);
+  return LD_OK;
+}
+
+int ld_rfft(ld_ctx* ctx, const void* u, void* fu, int nfields) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(u); LD_NONNULL(fu);
+  if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
+  return ctx->eng->rfft(u, fu, nfields);
+}
+
+int ld_irfft(ld_ctx* ctx, const void* fu, void* u, int nfields) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(fu); LD_NONNULL(u);
+  if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
+  return ctx->eng->irfft(fu, u, nfields);
+}
+
+int ld_forcing(ld_ctx* ctx, const void* uk, void* fk, double* ek_out) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(fk);
+  return ctx->eng->forcing(uk, fk, ek_out);
+}
+
+int ld_clip_zero(ld_ctx* ctx, void* x, void* out) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(x); LD_NONNULL(out);
+  return ctx->eng->clip_zero(x, out);
+}
+
+int ld_full_rhs(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np, void* ku, void* kn,
+                double* kp, void* fc, double visc, double forc, void* u_out, void* v_out) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n); LD_NONNULL(ku); LD_NONNULL(kn); LD_NONNULL(fc);
+  if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
+  if (np > 0) { LD_NONNULL(coord); LD_NONNULL(kp); }
+  return ctx->eng->full_rhs(uk, n, coord, np, ku, kn, kp, fc, visc, forc, u_out, v_out);
+}
+
+int ld_rk4(ld_ctx* ctx, double h, const void* uk, void* n, const double* coord, int64_t np, void* uknew, void* nnew,
+           double* coord_new) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n); LD_NONNULL(uknew); LD_NONNULL(nnew);
+  if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
+  if (np > 0) { LD_NONNULL(coord); LD_NONNULL(coord_new); }
+  return ctx->eng->rk4(h, uk, n, coord, np, uknew, nnew, coord_new);
+}
+
+int ld_state_set(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n);
+  if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
+  if (np > 0) LD_NONNULL(coord);
+  return ctx->eng->state_set(uk, n, coord, np);
+}
+
+int ld_state_get(ld_ctx* ctx, void* uk, void* n, double* coord) {
+  LD_CHECK_CTX(ctx);
+  return ctx->eng->state_get(uk, n, coord);
+}
+
+int ld_advance(ld_ctx* ctx, double h, int32_t nsteps) {
+  LD_CHECK_CTX(ctx);
+  if (nsteps < 0) { set_error("nsteps must be >= 0"); return LD_ERR_ARG; }
+  return ctx->eng->advance(h, nsteps);
+}
+
+int ld_step(ld_ctx* ctx, double h, void* uk, void* n, double* coord, int64_t np, int32_t nsteps) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n);
+  if (np < 0 || nsteps < 0) { set_error("np and nsteps must be >= 0"); return LD_ERR_ARG; }
+  if (np > 0) LD_NONNULL(coord);
+  int rc = ctx->eng->state_set(uk, n, coord, np);
+  if (rc != LD_OK) return rc;
+  const int adv = ctx->eng->advance(h, nsteps);
+  if (adv != LD_OK && adv != LD_ERR_BLOWUP) return adv;
+  rc = ctx->eng->state_get(uk, n, coord);
+  if (rc != LD_OK) return rc;
+  return adv;
+}
+
+int ld_status(ld_ctx* ctx, double* last_max, int32_t* blown_up) {
+  LD_CHECK_CTX(ctx);
+  return ctx->eng->status(last_max, blown_up);
+}
+
+int ld_interp_cosine(ld_ctx* ctx, const double* coord, int64_t np, const void* fields, int32_t ncomp, double* out) {
+  LD_CHECK_CTX(ctx);
+  if (np < 0 || ncomp < 1) { set_error("np must be >= 0 and ncomp >= 1"); return LD_ERR_ARG; }
+  if (np > 0) { LD_NONNULL(coord); LD_NONNULL(fields); LD_NONNULL(out); }
+  if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
+  return ctx->eng->interp(coord, np, fields, ncomp, out);
+}
+
+int ld_exterp_cosine_scalar(ld_ctx* ctx, const double* coord, int64_t np, const double* w, void* c) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(c);
+  if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
+  if (np > 0) { LD_NONNULL(coord); LD_NONNULL(w); }
+  if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
+  return ctx->eng->exterp(coord, np, w, c);
+}
+
+int ld_prhs(ld_ctx* ctx, const double* coord, int64_t np, const void* u, const void* us, const void* n, void* fc,
+            double* rhs, double* interpmat, double* exterpmat) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(u); LD_NONNULL(us); LD_NONNULL(n); LD_NONNULL(fc);
+  if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
+  if (np > 0) { LD_NONNULL(coord); LD_NONNULL(rhs); LD_NONNULL(interpmat); LD_NONNULL(exterpmat); }
+  if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
+  return ctx->eng->prhs(coord, np, u, us, n, fc, rhs, interpmat, exterpmat);
+}
+
+}  // extern "C"

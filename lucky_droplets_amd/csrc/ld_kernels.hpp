@@ -1,0 +1,768 @@
+// Kernels of the forced-DNS + droplet RK4 step (templated on precision T and
+// grid size N).  Layouts follow the reference at one MPI rank:
+//   physical field  [x][y][z]   (z contiguous), N^3 reals
+//   spectral field  [kx][ky][kz] (kz contiguous), N*N*Nf complex, Nf = N/2+1
+// A 3-D r2c transform is three line passes: z (real <-> complex, contiguous
+// lines), y (stride Nf) and x (stride N*Nf); the inverse runs x, y, z.
+// Pointwise work of the step is fused into the first/last pass that touches
+// the data (see each kernel's comment for the reference lines it covers).
+#pragma once
+#include "ld_fft.hpp"
+
+namespace ld {
+
+// ===================================================================== geometry
+template <int N> struct Geo {
+  static constexpr int Nf = N / 2 + 1;
+  static constexpr int NNf = N * Nf;           // complex elements per plane (x fixed)
+  static constexpr size_t C = (size_t)N * N * Nf;   // complex elements per field
+  static constexpr size_t R = (size_t)N * N * N;    // reals per field
+};
+
+// Loads/stores at a 32-bit element index from a uniform base pointer: the
+// byte offset stays 32-bit so the address is SGPR base + VGPR offset.
+template <typename X>
+__device__ __forceinline__ X ld32(const X* base, uint32_t idx) {
+  return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(base) + (size_t)(idx * (uint32_t)sizeof(X)));
+}
+template <typename X>
+__device__ __forceinline__ void st32(X* base, uint32_t idx, X v) {
+  *reinterpret_cast<X*>(reinterpret_cast<char*>(base) + (size_t)(idx * (uint32_t)sizeof(X))) = v;
+}
+
+// Strided passes: AXIS 0 = x (lines q in [0, N*Nf), element stride N*Nf),
+// AXIS 1 = y (lines q = x*Nf + kz, element stride Nf).
+template <int N, int AXIS> __device__ __forceinline__ int lineBase(int q) {
+  constexpr int Nf = N / 2 + 1;
+  if constexpr (AXIS == 0) return q;
+  else return (q / Nf) * (N * Nf) + (q % Nf);
+}
+template <int N, int AXIS> struct JS { static constexpr int v = AXIS == 0 ? N * (N / 2 + 1) : (N / 2 + 1); };
+
+// Spectral-space operator values at one mode (forced-dns-sm.py:125-136).
+template <typename T> struct Mode {
+  T kx, ky, kz;   // wavenumbers
+  T k2;           // |k|^2 (exact integer)
+  T mask;         // dealias: kint <= N/3
+  T invlap;       // mask / lap  (0 at k = 0)
+  int kint;       // clip(round(|k|), N/2)
+};
+template <typename T, int N>
+__device__ __forceinline__ Mode<T> mode(int ix, int iy, int iz) {
+  Mode<T> m;
+  const int kx = kwave(ix, N), ky = kwave(iy, N), kz = iz;
+  const int k2 = kx * kx + ky * ky + kz * kz;
+  int ki = kround(k2);
+  if (ki > N / 2) ki = N / 2;
+  m.kx = (T)kx; m.ky = (T)ky; m.kz = (T)kz; m.k2 = (T)k2; m.kint = ki;
+  m.mask = (3 * ki <= N) ? (T)1 : (T)0;
+  m.invlap = (k2 == 0) ? (T)0 : m.mask / (T)(-k2);
+  return m;
+}
+
+// ================================================================ block reductions
+// Deterministic sum over the block (fixed tree order); result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double blockSum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+template <int NT>
+__device__ __forceinline__ double blockMax(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] = fmax(sh[tid], sh[tid + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// ======================================================== plain strided pass
+// One line pass along x or y for `gridDim.y` fields (field stride fstride).
+// DIR < 0: forward (input natural order, output natural order);
+// DIR > 0: inverse times `scale`.  In place is allowed (each block owns its lines).
+template <typename T, int N, int M, int AXIS, int DIR>
+__global__ void __launch_bounds__(M * Plan<N>::N1)
+k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, js = JS<N, AXIS>::v;
+  __shared__ C<T> lds[M * (N + 1)];
+  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
+  const int q = blockIdx.x * M + m;
+  const C<T>* src = in + (size_t)blockIdx.y * fstride;
+  C<T>* dst = out + (size_t)blockIdx.y * fstride;
+  const uint32_t lb = (uint32_t)lineBase<N, AXIS>(q);
+  C<T>* L = lds + m * (N + 1);
+  C<T> a[N2];
+  if constexpr (DIR < 0) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(src, lb + (uint32_t)(posIn<N>(n1, n2) * js));
+    lineFwd<T, N>(a, L, n1, true, tw);
+#pragma unroll
+    for (int r = 0; r < N2; ++r) st32(dst, lb + (uint32_t)(posOut<N>(n1, r) * js), a[r]);
+  } else {
+#pragma unroll
+    for (int r = 0; r < N2; ++r) a[r] = ld32(src, lb + (uint32_t)(posOut<N>(n1, r) * js));
+    lineInv<T, N>(a, L, n1, true, tw);
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) st32(dst, lb + (uint32_t)(posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
+  }
+}
+
+// ============================================================= z-pass machinery
+// A block owns P consecutive (x,y) lines of several fields.  Real lines are
+// paired two-for-one into complex lines: lines (f, 2c) and (f, 2c+1) share one
+// complex FFT.  Each pair lives in an LDS "region" of 2*Nf complex elements,
+// which holds the pair's spectral lines, the FFT exchange and the two real
+// lines in turn.
+template <typename T, int N, int P>
+struct ZTile {
+  static constexpr int Nf = N / 2 + 1;
+  static constexpr int REG = 2 * Nf;
+  C<T>* lds;
+  __device__ __forceinline__ C<T>* region(int r) const { return lds + r * REG; }
+  __device__ __forceinline__ T& re(int f, int p, int j) const {
+    T* Rr = (T*)region(f * (P / 2) + p / 2);
+    return Rr[(p & 1) * N + j];
+  }
+  __device__ __forceinline__ C<T>& spec(int f, int p, int k) const {
+    return region(f * (P / 2) + p / 2)[(p & 1) * Nf + k];
+  }
+};
+
+// Inverse z transform (c2r, unnormalised) of FIN fields already loaded as
+// spectral lines into the tile; leaves real lines in the tile.  Imaginary
+// parts of the kz = 0 and kz = N/2 modes are ignored, as a c2r transform does.
+template <typename T, int N, int P, int FIN>
+__device__ __forceinline__ void zInverse(const ZTile<T, N, P>& tile, const C<T>* __restrict__ tw) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1;
+  const int r = threadIdx.x / N1, n1 = threadIdx.x % N1;
+  const bool active = r < FIN * P / 2;
+  C<T> a[N2];
+  C<T>* reg = tile.region(active ? r : 0);
+  if (active) {
+    const C<T>* Xa = reg;        // even line
+    const C<T>* Xb = reg + Nf;   // odd line
+#pragma unroll
+    for (int rr = 0; rr < N2; ++rr) {
+      const int j = posOut<N>(n1, rr);
+      C<T> A, B;
+      if (j <= N / 2) {
+        A = Xa[j]; B = Xb[j];
+        if (j == 0 || j == N / 2) { A.y = (T)0; B.y = (T)0; }
+      } else {
+        A = cconj<T>(Xa[N - j]); B = cconj<T>(Xb[N - j]);
+      }
+      a[rr] = mkc<T>(A.x - B.y, A.y + B.x);   // A + i B
+    }
+  }
+  lineInv<T, N>(a, reg, n1, active, tw);
+  __syncthreads();
+  if (active) {
+    T* Rr = (T*)reg;
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      const int j = posIn<N>(n1, n2);
+      Rr[j] = a[n2].x;
+      Rr[N + j] = a[n2].y;
+    }
+  }
+  __syncthreads();
+}
+
+// Forward z transform (r2c, unnormalised) of FOUT real fields in the tile;
+// leaves each pair's complex spectrum Z (natural order, N entries) in its
+// region.  Split with zSplit().
+template <typename T, int N, int P, int FOUT>
+__device__ __forceinline__ void zForward(const ZTile<T, N, P>& tile, const C<T>* __restrict__ tw) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2;
+  const int r = threadIdx.x / N1, n1 = threadIdx.x % N1;
+  const bool active = r < FOUT * P / 2;
+  C<T> a[N2];
+  C<T>* reg = tile.region(active ? r : 0);
+  if (active) {
+    const T* Rr = (const T*)reg;
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      const int j = posIn<N>(n1, n2);
+      a[n2] = mkc<T>(Rr[j], Rr[N + j]);
+    }
+  }
+  lineFwd<T, N>(a, reg, n1, active, tw);
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int rr = 0; rr < N2; ++rr) reg[posOut<N>(n1, rr)] = a[rr];
+  }
+  __syncthreads();
+}
+
+// Spectrum of real line p (of field f) at kz = k from the pair's complex FFT.
+template <typename T, int N, int P>
+__device__ __forceinline__ C<T> zSplit(const ZTile<T, N, P>& tile, int f, int p, int k) {
+  const C<T>* Z = tile.region(f * (P / 2) + p / 2);
+  const C<T> zk = Z[k];
+  const C<T> zm = cconj<T>(Z[(N - k) & (N - 1)]);
+  if ((p & 1) == 0) return mkc<T>((T)0.5 * (zk.x + zm.x), (T)0.5 * (zk.y + zm.y));
+  // -i/2 (zk - zm)
+  return mkc<T>((T)0.5 * (zk.y - zm.y), (T)-0.5 * (zk.x - zm.x));
+}
+
+template <int FIN, int FOUT> struct ZMax { static constexpr int v = FIN > FOUT ? FIN : FOUT; };
+template <int N, int P, int FIN, int FOUT> struct ZThreads {
+  static constexpr int v = ZMax<FIN, FOUT>::v * P / 2 * Plan<N>::N1;
+};
+
+// Generic z-pass kernel: [inverse of FIN spectral fields] -> Op::physical on
+// the P real lines of every field -> [forward of FOUT fields, stored through
+// Op::storeSpec].  Op::loadSpec supplies the spectral inputs.
+template <typename T, int N, int P, int FIN, int FOUT, class Op>
+__global__ void __launch_bounds__((ZThreads<N, P, FIN, FOUT>::v))
+k_z(Op op, const C<T>* __restrict__ tw) {
+  constexpr int Nf = N / 2 + 1;
+  constexpr int NREG = ZMax<FIN, FOUT>::v * P / 2;
+  __shared__ C<T> lds[NREG * 2 * Nf];
+  ZTile<T, N, P> tile{lds};
+  const int line0 = blockIdx.x * P;
+  const int nt = blockDim.x;
+  if constexpr (FIN > 0) {
+    for (int e = threadIdx.x; e < FIN * P * Nf; e += nt) {
+      const int f = e / (P * Nf), rem = e % (P * Nf);
+      const int p = rem / Nf, k = rem % Nf;
+      tile.spec(f, p, k) = op.loadSpec(f, line0 + p, k);
+    }
+    __syncthreads();
+    zInverse<T, N, P, FIN>(tile, tw);
+  }
+  op.physical(tile, line0);
+  __syncthreads();
+  if constexpr (FOUT > 0) {
+    zForward<T, N, P, FOUT>(tile, tw);
+    for (int e = threadIdx.x; e < FOUT * P * Nf; e += nt) {
+      const int f = e / (P * Nf), rem = e % (P * Nf);
+      const int p = rem / Nf, k = rem % Nf;
+      op.storeSpec(f, line0 + p, k, zSplit<T, N, P>(tile, f, p, k));
+    }
+  }
+}
+
+// ------------------------------------------------------------- z-pass ops
+// Plain c2r along z of F fields, times `scale` (last pass of irfft_mpi,
+// forced-dns-sm.py:212).
+template <typename T, int N, int P, int F>
+struct OpZInv {
+  const C<T>* in; T* out; size_t fin, fout; T scale;
+  __device__ C<T> loadSpec(int f, int line, int k) const { return in[f * fin + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
+    for (int e = threadIdx.x; e < F * P * N; e += blockDim.x) {
+      const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
+      out[f * fout + (size_t)(line0 + p) * N + j] = t.re(f, p, j) * scale;
+    }
+  }
+  __device__ void storeSpec(int, int, int, C<T>) const {}
+};
+// Plain r2c along z of F fields (first pass of rfft_mpi, forced-dns-sm.py:202).
+template <typename T, int N, int P, int F>
+struct OpZFwd {
+  const T* in; C<T>* out; size_t fin, fout;
+  __device__ C<T> loadSpec(int, int, int) const { return C<T>{}; }
+  __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
+    for (int e = threadIdx.x; e < F * P * N; e += blockDim.x) {
+      const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
+      t.re(f, p, j) = in[f * fin + (size_t)(line0 + p) * N + j];
+    }
+  }
+  __device__ void storeSpec(int f, int line, int k, C<T> v) const { out[f * fout + (size_t)line * (N / 2 + 1) + k] = v; }
+};
+// c2r followed by r2c of 3 fields (the reality round trip of
+// forced-dns-sm.py:597-603), in place.
+template <typename T, int N, int P>
+struct OpZRound {
+  C<T>* buf; size_t fs; T scale;
+  __device__ C<T> loadSpec(int f, int line, int k) const { return buf[f * fs + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ void physical(const ZTile<T, N, P>& t, int) const {
+    for (int e = threadIdx.x; e < 3 * P * N; e += blockDim.x) {
+      const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
+      t.re(f, p, j) *= scale;
+    }
+  }
+  __device__ void storeSpec(int f, int line, int k, C<T> v) const { buf[f * fs + (size_t)line * (N / 2 + 1) + k] = v; }
+};
+
+// Velocity and vorticity to physical space, u stored, then the rotational
+// nonlinear term u x omega transformed along z (forced-dns-sm.py:291-301, and
+// the first pass of the r2c at :304-306).  Spectral input: 6 fields
+// (u_x,u_y,u_z,w_x,w_y,w_z) already inverse-transformed along x and y; output:
+// 3 fields written over inputs 0..2.
+template <typename T, int N, int P>
+struct OpZCross {
+  C<T>* w; size_t fs; T* u; T scale;
+  __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
+    constexpr size_t R = (size_t)N * N * N;
+    for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
+      const int p = e / N, j = e % N;
+      const T ux = t.re(0, p, j) * scale, uy = t.re(1, p, j) * scale, uz = t.re(2, p, j) * scale;
+      const T wx = t.re(3, p, j) * scale, wy = t.re(4, p, j) * scale, wz = t.re(5, p, j) * scale;
+      const size_t o = (size_t)(line0 + p) * N + j;
+      u[o] = ux; u[R + o] = uy; u[2 * R + o] = uz;
+      t.re(0, p, j) = wz * uy - wy * uz;
+      t.re(1, p, j) = wx * uz - wz * ux;
+      t.re(2, p, j) = wy * ux - wx * uy;
+    }
+  }
+  __device__ void storeSpec(int f, int line, int k, C<T> v) const { w[f * fs + (size_t)line * (N / 2 + 1) + k] = v; }
+};
+
+// Stage density as the stage sees it: the reference stores clip_zero(x) =
+// clip(x, 0) * oldmean / newmean (forced-dns-sm.py:278-285); we keep x and
+// apply the clip lazily.  clip[0] = oldmean, clip[1] = newmean, clip[2] = 1
+// to apply, 0 to pass x through unchanged.
+template <typename T>
+__device__ __forceinline__ T nStage(T x, const double* __restrict__ clip) {
+  if (clip[2] != 0.0) {
+    const double xc = x > (T)0 ? (double)x : 0.0;
+    return (T)((xc * clip[0]) / clip[1]);
+  }
+  return x;
+}
+
+// Slow-manifold velocity v to physical space (forced-dns-sm.py:319-321, last
+// pass), v stored, then n*v transformed along z (:326-329, first pass).
+template <typename T, int N, int P>
+struct OpZVel {
+  C<T>* w; size_t fs; T* v; const T* n; const double* clip; T scale; T tpsg;
+  __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
+    constexpr size_t R = (size_t)N * N * N;
+    for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
+      const int p = e / N, j = e % N;
+      const size_t o = (size_t)(line0 + p) * N + j;
+      const T vx = t.re(0, p, j) * scale, vy = t.re(1, p, j) * scale;
+      const T vz = t.re(2, p, j) * scale - tpsg;
+      v[o] = vx; v[R + o] = vy; v[2 * R + o] = vz;
+      const T nn = nStage<T>(n[o], clip);
+      t.re(0, p, j) = nn * vx;
+      t.re(1, p, j) = nn * vy;
+      t.re(2, p, j) = nn * vz;
+    }
+  }
+  __device__ void storeSpec(int f, int line, int k, C<T> v_) const { w[f * fs + (size_t)line * (N / 2 + 1) + k] = v_; }
+};
+
+// Density RHS kn to physical space (last pass of forced-dns-sm.py:330) fused
+// with the RK4 update of n (:343-360) and the two sums of the next clip_zero
+// (:281-283).  Modes:
+//   kn_out != null        : store kn (full_RHS semantics, no RK);
+//   nacc / nnext          : nacc = (acc_init ? n0 : nacc) + wacc*kn,
+//                           nnext = max(n0,0) + anext*kn   (n0 = RK base)
+//   sum_target            : 0 -> sums of nnext, 1 -> sums of nacc, -1 none
+template <typename T, int N, int P>
+struct OpZKn {
+  const C<T>* w; size_t fs; T scale;
+  T* kn_out;
+  const T* n0; T* nacc; T* nnext;
+  T wacc, anext; int acc_init;
+  int sum_target; double* partials;
+  __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
+    double s_old = 0.0, s_new = 0.0;
+    for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
+      const int p = e / N, j = e % N;
+      const size_t o = (size_t)(line0 + p) * N + j;
+      const T kn = t.re(0, p, j) * scale;
+      if (kn_out) kn_out[o] = kn;
+      if (nacc) {
+        const T base = n0[o];
+        const T a = (acc_init ? base : nacc[o]) + wacc * kn;
+        nacc[o] = a;
+        if (sum_target == 1) { s_old += (double)a; s_new += a > (T)0 ? (double)a : 0.0; }
+        if (nnext) {
+          const T b = (base > (T)0 ? base : (T)0) + kn * anext;
+          nnext[o] = b;
+          if (sum_target == 0) { s_old += (double)b; s_new += b > (T)0 ? (double)b : 0.0; }
+        }
+      }
+    }
+    if (sum_target >= 0) {
+      constexpr int NT = ZThreads<N, P, 1, 0>::v;
+      __shared__ double sh[NT];
+      const double a = blockSum<NT>(s_old, sh);
+      if (threadIdx.x == 0) partials[2 * blockIdx.x] = a;
+      __syncthreads();
+      const double b = blockSum<NT>(s_new, sh);
+      if (threadIdx.x == 0) partials[2 * blockIdx.x + 1] = b;
+    }
+  }
+  __device__ void storeSpec(int, int, int, C<T>) const {}
+};
+
+// ======================================================== fused x passes
+// Velocity and vorticity (forced-dns-sm.py:291-297): x-inverse of u_hat and of
+// i k x u_hat.  Output: 6 fields (u_x,u_y,u_z,w_x,w_y,w_z), x-inverse-transformed.
+template <typename T, int N, int M>
+__global__ void __launch_bounds__(M * Plan<N>::N1)
+k_x_uw(const C<T>* __restrict__ uk, C<T>* __restrict__ out, size_t fs, const C<T>* __restrict__ tw, T scale) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
+  __shared__ C<T> lds[M * (N + 1)];
+  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
+  const int q = blockIdx.x * M + m;
+  const T ky = (T)kwave(q / Nf, N), kz = (T)(q % Nf);
+  C<T>* L = lds + m * (N + 1);
+  const C<T>* U0 = uk; const C<T>* U1 = uk + fs; const C<T>* U2 = uk + 2 * fs;
+  C<T> u0[N2], u1[N2], u2[N2];
+#pragma unroll
+  for (int r = 0; r < N2; ++r) {
+    const uint32_t o = (uint32_t)(q + posOut<N>(n1, r) * js);
+    u0[r] = ld32(U0, o); u1[r] = ld32(U1, o); u2[r] = ld32(U2, o);
+  }
+#pragma unroll 1
+  for (int comp = 0; comp < 6; ++comp) {
+    C<T> a[N2];
+#pragma unroll
+    for (int r = 0; r < N2; ++r) {
+      const T kx = (T)kwave(posOut<N>(n1, r), N);
+      C<T> s;
+      if (comp == 0) s = u0[r];
+      else if (comp == 1) s = u1[r];
+      else if (comp == 2) s = u2[r];
+      else if (comp == 3) s = cmuli<T>(mkc<T>(ky * u2[r].x - kz * u1[r].x, ky * u2[r].y - kz * u1[r].y));
+      else if (comp == 4) s = cmuli<T>(mkc<T>(kz * u0[r].x - kx * u2[r].x, kz * u0[r].y - kx * u2[r].y));
+      else s = cmuli<T>(mkc<T>(kx * u1[r].x - ky * u0[r].x, kx * u1[r].y - ky * u0[r].y));
+      a[r] = s;
+    }
+    lineInv<T, N>(a, L, n1, true, tw);
+    C<T>* O = out + comp * fs;
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) st32(O, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
+  }
+}
+
+// Arguments of the fused spectral RHS pass.
+template <typename T> struct RhsArgs {
+  C<T>* nl;            // in: u x w, forward-transformed along z and y (3 fields); out: v_hat x-inverse
+  const C<T>* us;      // stage input velocity u_hat (3)
+  const C<T>* ubase;   // RK base u_hat (3)
+  C<T>* acc;           // RK accumulator (3) or null
+  C<T>* usnext;        // next stage input (3) or null
+  C<T>* ku;            // raw RHS (3) or null
+  const double* fac;   // forcing factor per shell (Nf), times isforcing*forc
+  size_t fs;
+  T nu_eff, tps, wacc, anext, scale;
+  int acc_init;
+};
+
+// Nonlinear term + forcing + dealiasing + pressure projection + viscosity
+// (forced-dns-sm.py:239-275, 304-316), the RK4 accumulation of u_hat
+// (:338-360) and the slow-manifold velocity v_hat (:319-321) x-inverse
+// transformed for the next passes.
+template <typename T, int N, int M>
+__global__ void __launch_bounds__(M * Plan<N>::N1)
+k_x_rhs(RhsArgs<T> A, const C<T>* __restrict__ tw) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
+  __shared__ C<T> lds[M * (N + 1)];
+  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
+  const int q = blockIdx.x * M + m;
+  const int iy = q / Nf, iz = q % Nf;
+  C<T>* L = lds + m * (N + 1);
+  const size_t fs = A.fs;
+  C<T>* NL0 = A.nl; C<T>* NL1 = A.nl + fs; C<T>* NL2 = A.nl + 2 * fs;
+  const C<T>* US0 = A.us; const C<T>* US1 = A.us + fs; const C<T>* US2 = A.us + 2 * fs;
+  const C<T>* UB0 = A.ubase; const C<T>* UB1 = A.ubase + fs; const C<T>* UB2 = A.ubase + 2 * fs;
+  C<T> a0[N2], a1[N2], a2[N2];
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    const uint32_t o = (uint32_t)(q + posIn<N>(n1, n2) * js);
+    a0[n2] = ld32(NL0, o); a1[n2] = ld32(NL1, o); a2[n2] = ld32(NL2, o);
+  }
+  lineFwd<T, N>(a0, L, n1, true, tw);
+  lineFwd<T, N>(a1, L, n1, true, tw);
+  lineFwd<T, N>(a2, L, n1, true, tw);
+#pragma unroll
+  for (int r = 0; r < N2; ++r) {
+    const int ix = posOut<N>(n1, r);
+    const Mode<T> md = mode<T, N>(ix, iy, iz);
+    const uint32_t o = (uint32_t)(q + ix * js);
+    const C<T> us0 = ld32(US0, o), us1 = ld32(US1, o), us2 = ld32(US2, o);
+    // forcing (forced-dns-sm.py:262-275)
+    const T F = (T)A.fac[md.kint];
+    C<T> f0 = cscale<T>(us0, F * md.mask), f1 = cscale<T>(us1, F * md.mask), f2 = cscale<T>(us2, F * md.mask);
+    const C<T> dv = mkc<T>(md.kx * f0.x + md.ky * f1.x + md.kz * f2.x, md.kx * f0.y + md.ky * f1.y + md.kz * f2.y);
+    const C<T> pf = cscale<T>(dv, md.invlap * md.mask);
+    f0 = cscale<T>(cadd<T>(f0, cscale<T>(pf, md.kx)), md.mask);
+    f1 = cscale<T>(cadd<T>(f1, cscale<T>(pf, md.ky)), md.mask);
+    f2 = cscale<T>(cadd<T>(f2, cscale<T>(pf, md.kz)), md.mask);
+    // dealiased nonlinear term + forcing (:304-306)
+    const C<T> R0 = cscale<T>(cadd<T>(a0[r], f0), md.mask);
+    const C<T> R1 = cscale<T>(cadd<T>(a1[r], f1), md.mask);
+    const C<T> R2 = cscale<T>(cadd<T>(a2[r], f2), md.mask);
+    // pressure pk = i invlap (k . R)  (:309)
+    const C<T> D = mkc<T>(md.kx * R0.x + md.ky * R1.x + md.kz * R2.x, md.kx * R0.y + md.ky * R1.y + md.kz * R2.y);
+    const C<T> pk = cmuli<T>(cscale<T>(D, md.invlap));
+    // i k_c pk and viscous term nu k^2 u  (:314-316)
+    const C<T> ikp0 = cmuli<T>(cscale<T>(pk, md.kx)), ikp1 = cmuli<T>(cscale<T>(pk, md.ky)), ikp2 = cmuli<T>(cscale<T>(pk, md.kz));
+    const T nk2 = A.nu_eff * md.k2;
+    const C<T> vs0 = cscale<T>(us0, nk2), vs1 = cscale<T>(us1, nk2), vs2 = cscale<T>(us2, nk2);
+    const C<T> k0 = csub<T>(csub<T>(R0, ikp0), vs0);
+    const C<T> k1 = csub<T>(csub<T>(R1, ikp1), vs1);
+    const C<T> k2 = csub<T>(csub<T>(R2, ikp2), vs2);
+    if (A.ku) { st32(A.ku, o, k0); st32(A.ku + fs, o, k1); st32(A.ku + 2 * fs, o, k2); }
+    if (A.acc) {
+      C<T> c0, c1, c2;
+      if (A.acc_init) { c0 = ld32(UB0, o); c1 = ld32(UB1, o); c2 = ld32(UB2, o); }
+      else { c0 = ld32(A.acc, o); c1 = ld32(A.acc + fs, o); c2 = ld32(A.acc + 2 * fs, o); }
+      st32(A.acc, o, cadd<T>(c0, cscale<T>(k0, A.wacc)));
+      st32(A.acc + fs, o, cadd<T>(c1, cscale<T>(k1, A.wacc)));
+      st32(A.acc + 2 * fs, o, cadd<T>(c2, cscale<T>(k2, A.wacc)));
+    }
+    if (A.usnext) {
+      const C<T> b0 = ld32(UB0, o), b1 = ld32(UB1, o), b2 = ld32(UB2, o);
+      st32(A.usnext, o, cadd<T>(b0, cscale<T>(k0, A.anext)));
+      st32(A.usnext + fs, o, cadd<T>(b1, cscale<T>(k1, A.anext)));
+      st32(A.usnext + 2 * fs, o, cadd<T>(b2, cscale<T>(k2, A.anext)));
+    }
+    // v_hat = (u - tps (-i k p - nu k^2 u)) * mask  (:319-321)
+    a0[r] = cscale<T>(csub<T>(us0, cscale<T>(csub<T>(cscale<T>(ikp0, (T)-1), vs0), A.tps)), md.mask);
+    a1[r] = cscale<T>(csub<T>(us1, cscale<T>(csub<T>(cscale<T>(ikp1, (T)-1), vs1), A.tps)), md.mask);
+    a2[r] = cscale<T>(csub<T>(us2, cscale<T>(csub<T>(cscale<T>(ikp2, (T)-1), vs2), A.tps)), md.mask);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  lineInv<T, N>(a0, L, n1, true, tw);
+  lineInv<T, N>(a1, L, n1, true, tw);
+  lineInv<T, N>(a2, L, n1, true, tw);
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    const uint32_t o = (uint32_t)(q + posIn<N>(n1, n2) * js);
+    st32(NL0, o, cscale<T>(a0[n2], A.scale));
+    st32(NL1, o, cscale<T>(a1[n2], A.scale));
+    st32(NL2, o, cscale<T>(a2[n2], A.scale));
+  }
+}
+
+// Density RHS in spectral space (forced-dns-sm.py:325-330):
+//   kn_hat = (-i k.(nv)_hat*mask - fc_hat*mask) * mask, then x-inverse.
+// In: 4 fields (nv_x, nv_y, nv_z, fc) forward-transformed along z and y;
+// out: field 0 (x-inverse of kn_hat).
+template <typename T, int N, int M>
+__global__ void __launch_bounds__(M * Plan<N>::N1)
+k_x_kn(C<T>* w, size_t fs, const C<T>* __restrict__ tw, T scale) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
+  __shared__ C<T> lds[M * (N + 1)];
+  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
+  const int q = blockIdx.x * M + m;
+  const int iy = q / Nf, iz = q % Nf;
+  C<T>* L = lds + m * (N + 1);
+  C<T> acc[N2], a[N2];
+#pragma unroll 1
+  for (int f = 0; f < 4; ++f) {
+    const C<T>* W = w + f * fs;
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(W, (uint32_t)(q + posIn<N>(n1, n2) * js));
+    lineFwd<T, N>(a, L, n1, true, tw);
+#pragma unroll
+    for (int r = 0; r < N2; ++r) {
+      const Mode<T> md = mode<T, N>(posOut<N>(n1, r), iy, iz);
+      const C<T> v = cscale<T>(a[r], md.mask);
+      if (f == 0) acc[r] = cscale<T>(v, md.kx);
+      else if (f == 1) acc[r] = cadd<T>(acc[r], cscale<T>(v, md.ky));
+      else if (f == 2) acc[r] = cadd<T>(acc[r], cscale<T>(v, md.kz));
+      else {
+        const C<T> mi = mkc<T>(acc[r].y, -acc[r].x);   // -i * (k . nv)
+        acc[r] = cscale<T>(csub<T>(mi, v), md.mask);
+      }
+    }
+  }
+  lineInv<T, N>(acc, L, n1, true, tw);
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) st32(w, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(acc[n2], scale));
+}
+
+// Last pass of the reality round trip + solenoidal projection
+// (forced-dns-sm.py:601-610) + blow-up statistic max Re(u_hat) (:618).
+template <typename T, int N, int M>
+__global__ void __launch_bounds__(M * Plan<N>::N1)
+k_x_proj(const C<T>* w, C<T>* uk, size_t fs, const C<T>* __restrict__ tw, double* maxpart) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
+  constexpr int NT = M * N1;
+  __shared__ C<T> lds[M * (N + 1)];
+  __shared__ double sh[NT];
+  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
+  const int q = blockIdx.x * M + m;
+  const int iy = q / Nf, iz = q % Nf;
+  C<T>* L = lds + m * (N + 1);
+  C<T> a0[N2], a1[N2], a2[N2];
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    const uint32_t o = (uint32_t)(q + posIn<N>(n1, n2) * js);
+    a0[n2] = ld32(w, o); a1[n2] = ld32(w + fs, o); a2[n2] = ld32(w + 2 * fs, o);
+  }
+  lineFwd<T, N>(a0, L, n1, true, tw);
+  lineFwd<T, N>(a1, L, n1, true, tw);
+  lineFwd<T, N>(a2, L, n1, true, tw);
+  double mx = -INFINITY;
+#pragma unroll
+  for (int r = 0; r < N2; ++r) {
+    const int ix = posOut<N>(n1, r);
+    const Mode<T> md = mode<T, N>(ix, iy, iz);
+    const C<T> d = mkc<T>(md.kx * a0[r].x + md.ky * a1[r].x + md.kz * a2[r].x,
+                          md.kx * a0[r].y + md.ky * a1[r].y + md.kz * a2[r].y);
+    const C<T> pk = cscale<T>(d, md.invlap);
+    const C<T> b0 = cadd<T>(a0[r], cscale<T>(pk, md.kx));
+    const C<T> b1 = cadd<T>(a1[r], cscale<T>(pk, md.ky));
+    const C<T> b2 = cadd<T>(a2[r], cscale<T>(pk, md.kz));
+    const uint32_t o = (uint32_t)(q + ix * js);
+    st32(uk, o, b0); st32(uk + fs, o, b1); st32(uk + 2 * fs, o, b2);
+    mx = fmax(mx, fmax((double)b0.x, fmax((double)b1.x, (double)b2.x)));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const double bm = blockMax<NT>(mx, sh);
+  if (threadIdx.x == 0) maxpart[blockIdx.x] = bm;
+}
+
+// ======================================================== small kernels
+// Shell energies of the forced shells and the constant-power factors
+// (forced-dns-sm.py:243-254).  One block; loops over the modes with every
+// |component| <= smax, which contains every shell s <= smax.
+//   ek[s]  = sum over kint == s of 0.5 |u_hat|^2 normalize
+//   fac[s] = f0 / (2 ek[s]) for s in shells (0 if |ek[s]| < 1e-10), times fscale
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_shell(const C<T>* __restrict__ uk, size_t fs, int smax, const int* __restrict__ shells, int nsh,
+        double f0, double fscale, double norm0, double* __restrict__ ek, double* __restrict__ fac) {
+  constexpr int NT = 256;
+  __shared__ double sh[NT];
+  constexpr int Nf = N / 2 + 1;
+  const int w = 2 * smax + 1;
+  const int nmodes = w * w * (smax + 1);
+  double part[9];
+#pragma unroll
+  for (int s = 0; s < 9; ++s) part[s] = 0.0;
+  for (int e = threadIdx.x; e < nmodes; e += NT) {
+    const int kz = e % (smax + 1), rem = e / (smax + 1);
+    const int ky = rem % w - smax, kx = rem / w - smax;
+    const int k2 = kx * kx + ky * ky + kz * kz;
+    const int ki = kround(k2);
+    if (ki > smax || ki >= 9) continue;
+    const int ix = kx < 0 ? kx + N : kx, iy = ky < 0 ? ky + N : ky;
+    const size_t o = ((size_t)ix * N + iy) * Nf + kz;
+    const double nrm = (kz == 0 || kz == N / 2) ? norm0 : 2.0 * norm0;
+    double e3 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const C<T> v = uk[c * fs + o];
+      e3 += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    }
+    const double val = 0.5 * e3 * nrm;
+#pragma unroll
+    for (int s = 0; s < 9; ++s) if (s == ki) part[s] += val;
+  }
+  for (int s = 0; s <= smax && s < 9; ++s) {
+    const double tot = blockSum<NT>(part[s], sh);
+    if (threadIdx.x == 0) ek[s] = tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < nsh; ++i) {
+      const int s = shells[i];
+      const double e = ek[s];
+      fac[s] = (fabs(e) < 1e-10) ? 0.0 : fscale * (f0 / (2.0 * e));
+    }
+  }
+}
+
+// Two sums of clip_zero over a real field: sum(x) and sum(max(x, 0))
+// (forced-dns-sm.py:281, :283), per block.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sum2(const T* __restrict__ x, size_t n, double* __restrict__ partials) {
+  constexpr int NT = 256;
+  __shared__ double sh[NT];
+  const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const size_t b = blockIdx.x * chunk, e = (b + chunk < n) ? b + chunk : n;
+  double s0 = 0.0, s1 = 0.0;
+  for (size_t i = b + threadIdx.x; i < e; i += NT) {
+    const double v = (double)x[i];
+    s0 += v; s1 += v > 0.0 ? v : 0.0;
+  }
+  const double a = blockSum<NT>(s0, sh);
+  if (threadIdx.x == 0) partials[2 * blockIdx.x] = a;
+  __syncthreads();
+  const double c = blockSum<NT>(s1, sh);
+  if (threadIdx.x == 0) partials[2 * blockIdx.x + 1] = c;
+}
+
+// Final clip_zero means: clip = {sum/N^3, sum+/N^3, 1}.
+static __global__ void __launch_bounds__(256)
+k_clip_final(const double* __restrict__ partials, int nblk, double inv_n3, double* __restrict__ clip) {
+  constexpr int NT = 256;
+  __shared__ double sh[NT];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += NT) { s0 += partials[2 * i]; s1 += partials[2 * i + 1]; }
+  const double a = blockSum<NT>(s0, sh);
+  __syncthreads();
+  const double b = blockSum<NT>(s1, sh);
+  if (threadIdx.x == 0) { clip[0] = a * inv_n3; clip[1] = b * inv_n3; clip[2] = 1.0; }
+}
+
+// out = clip_zero(x) with x clipped in place when clip_x (forced-dns-sm.py:282-285).
+template <typename T>
+__global__ void k_clip_apply(T* x, T* out, size_t n, const double* __restrict__ clip, int clip_x) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const T v = x[i];
+    const T c = v > (T)0 ? v : (T)0;
+    if (clip_x) x[i] = c;
+    out[i] = (T)(((double)c * clip[0]) / clip[1]);
+  }
+}
+
+// Blow-up test: flag |= max Re(u_hat) > thresh (forced-dns-sm.py:618); the
+// per-step maxima are kept for diagnostics.
+static __global__ void k_max_final(const double* __restrict__ part, int nblk, double thresh, int* flag, double* lastmax) {
+  constexpr int NT = 256;
+  __shared__ double sh[NT];
+  double m = -INFINITY;
+  for (int i = threadIdx.x; i < nblk; i += NT) m = fmax(m, part[i]);
+  const double b = blockMax<NT>(m, sh);
+  if (threadIdx.x == 0) { *lastmax = b; if (b > thresh) *flag = 1; }
+}
+
+// forcing(uk, fk) as a standalone pass (forced-dns-sm.py:262-275), given the
+// per-shell factors from k_shell.
+template <typename T, int N>
+__global__ void k_forcing_apply(const C<T>* __restrict__ uk, C<T>* __restrict__ fk, size_t fs,
+                                const double* __restrict__ fac) {
+  constexpr int Nf = N / 2 + 1;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= fs) return;
+  const int iz = (int)(i % Nf), rem = (int)(i / Nf), iy = rem % N, ix = rem / N;
+  const Mode<T> md = mode<T, N>(ix, iy, iz);
+  const T F = (T)fac[md.kint];
+  C<T> f0 = cscale<T>(uk[i], F * md.mask), f1 = cscale<T>(uk[fs + i], F * md.mask), f2 = cscale<T>(uk[2 * fs + i], F * md.mask);
+  const C<T> dv = mkc<T>(md.kx * f0.x + md.ky * f1.x + md.kz * f2.x, md.kx * f0.y + md.ky * f1.y + md.kz * f2.y);
+  const C<T> pf = cscale<T>(dv, md.invlap * md.mask);
+  fk[i] = cscale<T>(cadd<T>(f0, cscale<T>(pf, md.kx)), md.mask);
+  fk[fs + i] = cscale<T>(cadd<T>(f1, cscale<T>(pf, md.ky)), md.mask);
+  fk[2 * fs + i] = cscale<T>(cadd<T>(f2, cscale<T>(pf, md.kz)), md.mask);
+}
+
+template <typename T>
+__global__ void k_fill(T* x, size_t n, T v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = v;
+}
+template <typename T>
+__global__ void k_relu(T* x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const T v = x[i];
+    if (!(v > (T)0)) x[i] = (T)0;
+  }
+}
+
+}  // namespace ld

@@ -1,0 +1,310 @@
+// Droplet kernels: cosine-kernel gather/deposit (particles.py:6-35, 435-577),
+// droplet RHS (particles.py:817-837) and its RK4 stage update
+// (forced-dns-sm.py:337-362 for the droplet rows).
+//
+// Engine state is structure-of-arrays: positions in FP64 (pos[c*cap + p],
+// unwrapped, as the reference accumulates them), velocity and mass in PT
+// (vm[c*cap + p], c = vx, vy, vz, m/M0).  One thread per droplet.
+#pragma once
+#include "ld_kernels.hpp"
+
+namespace ld {
+
+// The 4-point cosine weights of one axis: nodes i-1..i+2 with i = floor(x/dx),
+// w = (1 + cos((x - X_j) pi / (2 dx))) / 4 (particles.py:12, 16, 456).  X_j is
+// taken unwrapped (j*dx), which equals the reference's wrapped X[j mod N]
+// through the period 4*dx of the kernel (N % 4 == 0).
+__device__ __forceinline__ double cosW(double x, int j, double dx, double pi_2dx) {
+  return 0.25 * (1.0 + cos((x - (double)j * dx) * pi_2dx));
+}
+template <int N> __device__ __forceinline__ int wrapIdx(int j) {
+  int jj = j % N;
+  return jj < 0 ? jj + N : jj;
+}
+template <int N>
+__device__ __forceinline__ void cosWeights(double x, double dx, double pi_2dx, int idx[4], double w[4]) {
+  const int i = (int)floor(x / dx);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    w[a] = cosW(x, i - 1 + a, dx, pi_2dx);
+    idx[a] = wrapIdx<N>(i - 1 + a);
+  }
+}
+
+// Periodic wrap of one coordinate, x mod L with numpy's sign convention
+// (particles.py:269 `x[:, :d] %= L`).
+__device__ __forceinline__ double wrapL(double x, double L) {
+  double r = fmod(x, L);
+  if (r < 0.0) r += L;
+  // a non-finite position must not become an out-of-range stencil index
+  if (!(r >= 0.0 && r <= L)) r = 0.0;
+  return r;
+}
+
+// Geometry of one droplet's stencil.  The x weights are evaluated inside the
+// x loop (xa(), wxa()) so the loop need not be unrolled.
+template <int N> struct Stencil {
+  double x, dx, pi_2dx;
+  int i0;
+  int iy[4], iz[4];
+  double wy[4], wz[4];
+  __device__ __forceinline__ void set(const double xw[3], double dx_, double pi_2dx_) {
+    x = xw[0]; dx = dx_; pi_2dx = pi_2dx_;
+    i0 = (int)floor(x / dx) - 1;
+    cosWeights<N>(xw[1], dx, pi_2dx, iy, wy);
+    cosWeights<N>(xw[2], dx, pi_2dx, iz, wz);
+  }
+  __device__ __forceinline__ int xa(int a) const { return wrapIdx<N>(i0 + a); }
+  __device__ __forceinline__ double wxa(int a) const { return cosW(x, i0 + a, dx, pi_2dx); }
+};
+
+// Gather of NC fields f[c*R + idx] (particles.py:6-20, 454-509): for each x
+// node a, sum over the (y, z) nodes, then weight by w_x(a).
+template <typename T, int N, int NC>
+__device__ __forceinline__ void gatherN(const Stencil<N>& s, const T* __restrict__ f, size_t R, double out[NC]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = 0.0;
+#pragma unroll 1
+  for (int a = 0; a < 4; ++a) {
+    double t[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) t[c] = 0.0;
+    const size_t plane = (size_t)s.xa(a) * N;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const size_t row = (plane + s.iy[b]) * N;
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        const double wyz = s.wy[b] * s.wz[cc];
+        const size_t o = row + s.iz[cc];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) t[c] += (double)f[c * R + o] * wyz;
+      }
+    }
+    const double wx = s.wxa(a);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] += t[c] * wx;
+  }
+}
+
+// Gather of the stage density with the lazy clip (nStage) applied per node.
+template <typename T, int N>
+__device__ __forceinline__ double gatherDensity(const Stencil<N>& s, const T* __restrict__ n, const double* __restrict__ clip) {
+  double out = 0.0;
+  const bool doclip = clip[2] != 0.0;
+#pragma unroll 1
+  for (int a = 0; a < 4; ++a) {
+    double t = 0.0;
+    const size_t plane = (size_t)s.xa(a) * N;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const size_t row = (plane + s.iy[b]) * N;
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        double v = (double)n[row + s.iz[cc]];
+        if (doclip && v < 0.0) v = 0.0;
+        t += v * (s.wy[b] * s.wz[cc]);
+      }
+    }
+    out += t * s.wxa(a);
+  }
+  if (doclip) out = (out * clip[0]) / clip[1];
+  return out;
+}
+
+// Deposit W into c with the same kernel, divided by the cell volume
+// (particles.py:23-35, 529-575).  Atomic adds (the reference's prange loop
+// races on the same updates).
+template <typename T, int N>
+__device__ __forceinline__ void deposit(const Stencil<N>& s, double W, double inv_vol, T* c) {
+#pragma unroll 1
+  for (int a = 0; a < 4; ++a) {
+    const double wa = W * s.wxa(a);
+    const size_t plane = (size_t)s.xa(a) * N;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const double wab = wa * s.wy[b];
+      const size_t row = (plane + s.iy[b]) * N;
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        unsafeAtomicAdd(&c[row + s.iz[cc]], (T)(wab * s.wz[cc] * inv_vol));
+      }
+    }
+  }
+}
+
+struct DropPhys {
+  double L, dx, pi_2dx, inv_vol;
+  double factor, growth, st_s, tau_eta, g;
+};
+
+// Droplet RHS at one stage coordinate (particles.py:823-835).  xw: wrapped
+// position, vp: velocity, m: mass/M0.  I: the 7 interpolated components
+// [u, v, n].  Returns rhs[7] and W (= mdot / M0).
+__device__ __forceinline__ void dropletRhs(const DropPhys& P, const double vp[3], double m, const double I[7],
+                                           double rhs[7], double& W) {
+  const double st = pow(m / P.factor, 2.0 / 3.0);                          // :823
+  const double d0 = vp[0] - I[3], d1 = vp[1] - I[4], d2 = vp[2] - I[5];
+  const double nrm = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  const double q = 1.0 + sqrt(P.st_s / st);
+  W = P.growth * (q * q) * I[6] * nrm;                                      // :828
+  rhs[0] = vp[0]; rhs[1] = vp[1]; rhs[2] = vp[2];                           // :831
+  const double den = st * P.tau_eta;
+  rhs[3] = (I[0] - vp[0]) / den;                                            // :832
+  rhs[4] = (I[1] - vp[1]) / den;
+  rhs[5] = (I[2] - vp[2]) / den - P.g;                                      // :834
+  rhs[6] = W;                                                               // :835
+}
+
+// ------------------------------------------------------------ fused RK stage
+template <typename T, typename PT> struct PStage {
+  int np, cap, stage;          // stage 0..3
+  double* x0; PT* v0;          // RK base (unwrapped); final stage writes the new state here
+  double* sx; PT* sv;          // sump accumulator
+  double* kx; PT* kv;          // stage rhs (read: previous stage, write: this stage)
+  const T* u; const T* v; const T* n; const double* clip;
+  T* fc;
+  double ah, wh;               // a_s * h, w_s * h
+  DropPhys P;
+};
+
+// One RK4 stage of every droplet: stage coordinate X0 + a h k_prev
+// (forced-dns-sm.py:347, 352, 357), wrap (particles.py:269), gather of
+// [u, v, n] (:826), growth rate and deposit (:828-829), RHS (:831-835) and
+// sump accumulation (forced-dns-sm.py:342-358).
+template <typename T, typename PT, int N>
+__global__ void __launch_bounds__(256)
+k_particles_stage(PStage<T, PT> A) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.np) return;
+  constexpr size_t R = (size_t)N * N * N;
+  const int cap = A.cap;
+  double xs[3], vp[3], m;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double x = A.x0[c * cap + p];
+    double vv = (double)A.v0[c * cap + p];
+    if (A.stage > 0) {
+      x = x + A.ah * A.kx[c * cap + p];
+      vv = (double)(PT)(A.v0[c * cap + p] + (PT)A.ah * A.kv[c * cap + p]);
+    }
+    xs[c] = x; vp[c] = vv;
+  }
+  m = (double)A.v0[3 * cap + p];
+  if (A.stage > 0) m = (double)(PT)(A.v0[3 * cap + p] + (PT)A.ah * A.kv[3 * cap + p]);
+  double xw[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) xw[c] = wrapL(xs[c], A.P.L);
+  Stencil<N> s;
+  s.set(xw, A.P.dx, A.P.pi_2dx);
+  double I[7];
+  gatherN<T, N, 3>(s, A.u, R, I);
+  gatherN<T, N, 3>(s, A.v, R, I + 3);
+  I[6] = gatherDensity<T, N>(s, A.n, A.clip);
+  double rhs[7], W;
+  dropletRhs(A.P, vp, m, I, rhs, W);
+  deposit<T, N>(s, W, A.P.inv_vol, A.fc);
+  // sump (forced-dns-sm.py:337, 342-358); the last stage writes the new state
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double prev = A.stage == 0 ? A.x0[c * cap + p] : A.sx[c * cap + p];
+    const double nv = prev + A.wh * rhs[c];
+    if (A.stage == 3) A.x0[c * cap + p] = nv; else A.sx[c * cap + p] = nv;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const PT prev = A.stage == 0 ? A.v0[c * cap + p] : A.sv[c * cap + p];
+    const PT nv = prev + (PT)A.wh * (PT)rhs[3 + c];
+    if (A.stage == 3) A.v0[c * cap + p] = nv; else A.sv[c * cap + p] = nv;
+  }
+  if (A.stage < 3) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A.kx[c * cap + p] = rhs[c];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) A.kv[c * cap + p] = (PT)rhs[3 + c];
+  }
+}
+
+// ------------------------------------------------------------ API kernels
+// interp_cosine (particles.py:435-511) of NC fields at rows of coord (Np, 7).
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_interp(const double* __restrict__ coord, int np, const T* __restrict__ f, int ncomp, double* __restrict__ out,
+         DropPhys P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  constexpr size_t R = (size_t)N * N * N;
+  double xw[3];
+  for (int c = 0; c < 3; ++c) xw[c] = wrapL(coord[(size_t)p * 7 + c], P.L);
+  Stencil<N> s;
+  s.set(xw, P.dx, P.pi_2dx);
+  for (int c = 0; c < ncomp; ++c) {
+    double v;
+    gatherN<T, N, 1>(s, f + (size_t)c * R, R, &v);
+    out[(size_t)p * ncomp + c] = v;
+  }
+}
+
+// exterp_cosine_scalar (particles.py:514-577): deposit w[p] into c (c must be zeroed).
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_exterp(const double* __restrict__ coord, int np, const double* __restrict__ w, T* c, DropPhys P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double xw[3];
+  for (int k = 0; k < 3; ++k) xw[k] = wrapL(coord[(size_t)p * 7 + k], P.L);
+  Stencil<N> s;
+  s.set(xw, P.dx, P.pi_2dx);
+  deposit<T, N>(s, w[p], P.inv_vol, c);
+}
+
+// pRHS (particles.py:817-837) on rows of a stage coordinate (Np, 7): writes
+// rhs (Np, 7), interpmat (Np, 7), exterpmat (Np) and deposits into fc (zeroed).
+// n is used as given (clip = nullptr) or through the lazy clip.
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_prhs(const double* __restrict__ coord, int np, const T* __restrict__ u, const T* __restrict__ us,
+       const T* __restrict__ n, const double* __restrict__ clip, T* fc, double* __restrict__ rhs,
+       double* __restrict__ interp, double* __restrict__ exterp, DropPhys P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  constexpr size_t R = (size_t)N * N * N;
+  const double* row = coord + (size_t)p * 7;
+  double xw[3], vp[3];
+  for (int c = 0; c < 3; ++c) { xw[c] = wrapL(row[c], P.L); vp[c] = row[3 + c]; }
+  Stencil<N> s;
+  s.set(xw, P.dx, P.pi_2dx);
+  double I[7];
+  gatherN<T, N, 3>(s, u, R, I);
+  gatherN<T, N, 3>(s, us, R, I + 3);
+  I[6] = gatherDensity<T, N>(s, n, clip);
+  double r[7], W;
+  dropletRhs(P, vp, row[6], I, r, W);
+  deposit<T, N>(s, W, P.inv_vol, fc);
+  for (int c = 0; c < 7; ++c) { rhs[(size_t)p * 7 + c] = r[c]; interp[(size_t)p * 7 + c] = I[c]; }
+  exterp[p] = W;
+}
+
+// AoS (Np, 7) rows <-> engine SoA state.
+template <typename PT>
+__global__ void k_rows_to_soa(const double* __restrict__ rows, int np, int cap, double* pos, PT* vm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  for (int c = 0; c < 3; ++c) pos[c * cap + p] = rows[(size_t)p * 7 + c];
+  for (int c = 0; c < 4; ++c) vm[c * cap + p] = (PT)rows[(size_t)p * 7 + 3 + c];
+}
+template <typename PT>
+__global__ void k_soa_to_rows(const double* __restrict__ pos, const PT* __restrict__ vm, int np, int cap, double* rows) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  for (int c = 0; c < 3; ++c) rows[(size_t)p * 7 + c] = pos[c * cap + p];
+  for (int c = 0; c < 4; ++c) rows[(size_t)p * 7 + 3 + c] = (double)vm[c * cap + p];
+}
+
+// Stage coordinate rows X + a*kp (forced-dns-sm.py:347) for the API path.
+static __global__ void k_axpy_rows(const double* __restrict__ x, const double* __restrict__ k, double a, int n, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = x[i] + a * k[i];
+}
+
+}  // namespace ld

@@ -1,0 +1,155 @@
+"""GPU path (libld.so) vs the CPU FP64 oracle on identical seeded inputs.
+
+Tolerances: FP64 1e-10 relative (max-abs error over max-abs value, per
+field); FP32 2e-5 relative for single transforms and 1e-4 for whole RHS/step
+results (single-precision rounding of O(1e2) transforms)."""
+import numpy as np
+import pytest
+
+from helpers import by_id, inputs, make_drops_oracle, make_flow_oracle, min_image, relerr
+from lucky_droplets_amd.config import Config
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-10, "f32": 1e-4}
+
+
+def _t(x, dev, dtype):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev).to(dtype)
+
+
+def _cfg(N, prec, nparts=0, **kw):
+    base = dict(N=N, precision=prec, nu=0.02, nshells=2, profile="k4", n_particles=nparts,
+                st_dist=("uniform", 0.05, 1.0), gravity=True)
+    base.update(kw)
+    return Config(**base)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("N", [16, 32, 64, 128])
+def test_fft_matches_scipy(gpu, N, prec):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(N, prec)
+    d = ForcedDNS(cfg, max_particles=0)
+    G = make_flow_oracle(cfg).grid
+    rng = np.random.default_rng(N)
+    u = rng.standard_normal((3, N, N, N))
+    ref = np.stack([G.rfft_mpi(u[c]) for c in range(3)])
+    fu = d.zeros_spec(3)
+    d.rfft_mpi(_t(u, gpu, d.rdtype), fu)
+    tol = 1e-13 if prec == "f64" else 2e-6
+    assert relerr(fu.cpu().numpy(), ref) < tol
+    # inverse of a general (non-Hermitian) half spectrum: c2r reads Re of kz = 0, N/2
+    X = rng.standard_normal((2, N, N, N // 2 + 1)) + 1j * rng.standard_normal((2, N, N, N // 2 + 1))
+    refi = np.stack([G.irfft_mpi(X[c]) for c in range(2)])
+    uo = d.zeros_real(2)
+    d.irfft_mpi(_t(X, gpu, d.cdtype), uo)
+    assert relerr(uo.cpu().numpy(), refi) < tol
+    d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_forcing(gpu, prec):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(32, prec)
+    uk, n, coord, pid = inputs(cfg, seed=1)
+    fl = make_flow_oracle(cfg)
+    ref = fl.forcing(uk)
+    d = ForcedDNS(cfg, max_particles=0)
+    fk = d.zeros_spec(3)
+    ek = torch.zeros(cfg.N // 2 + 1, dtype=torch.float64, device=gpu)
+    d.forcing(_t(uk, gpu, d.cdtype), fk, ek)
+    assert relerr(fk.cpu().numpy(), ref) < (1e-12 if prec == "f64" else 1e-5)
+    e_ref = fl.grid.e3d_to_e1d(fl.grid.energy3d(uk))
+    assert relerr(ek.cpu().numpy()[1:3], e_ref[1:3]) < (1e-12 if prec == "f64" else 1e-5)
+    d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("nparts", [0, 40])
+def test_full_rhs(gpu, prec, nparts):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(32, prec, nparts)
+    uk, n, coord, pid = inputs(cfg, seed=2, n_noise=0.8)
+    fl = make_flow_oracle(cfg)
+    dr = make_drops_oracle(cfg, coord, pid) if nparts else None
+    ku_r, kn_r, kp_r, _ = fl.full_RHS(0.0, uk, n.copy(), coord.copy(), coord.copy(), dr)
+    d = ForcedDNS(cfg, max_particles=nparts)
+    u_out = d.zeros_real(3)
+    v_out = d.zeros_real(3)
+    fc = d.zeros_real()
+    ku, kn, kp, _ = d.full_RHS(0.0, _t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), None,
+                               _t(coord, gpu, torch.float64) if nparts else None, fc=fc, u_out=u_out, v_out=v_out)
+    tol = TOL[prec]
+    assert relerr(u_out.cpu().numpy(), fl.last["u"]) < tol
+    assert relerr(v_out.cpu().numpy(), fl.last["v"]) < tol
+    assert relerr(ku.cpu().numpy(), ku_r) < tol
+    assert relerr(kn.cpu().numpy(), kn_r) < tol
+    if nparts:
+        assert relerr(fc.cpu().numpy(), fl.last["fc"]) < tol
+        kp_id = by_id(kp.cpu().numpy(), pid)
+        kp_ref = by_id(kp_r, dr.prtclid)
+        for cols in (slice(0, 3), slice(3, 6), slice(6, 7)):
+            assert relerr(kp_id[:, cols], kp_ref[:, cols]) < tol
+    d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_rk4(gpu, prec):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    from lucky_droplets_amd.particles import MPI_particles
+    nparts = 40
+    cfg = _cfg(32, prec, nparts)
+    uk, n, coord, pid = inputs(cfg, seed=3, n_noise=0.8)
+    n = np.abs(n)
+    fl = make_flow_oracle(cfg)
+    dr = make_drops_oracle(cfg, coord, pid)
+    h = 0.01
+    sump_r, uknew_r, nnew_r = fl.RK4(0.0, h, dr, uk, n.copy())
+    d = ForcedDNS(cfg)
+    stb = MPI_particles(None, cfg.L, cfg.N, nparts, cfg.sts, 0.05, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0,
+                        dns=d, coord=coord, prtclid=pid)
+    sump, uknew, nnew = d.RK4(0.0, h, stb, _t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype))
+    tol = TOL[prec]
+    assert relerr(uknew.cpu().numpy(), uknew_r) < tol
+    assert relerr(nnew.cpu().numpy(), nnew_r) < tol
+    a = by_id(sump.cpu().numpy(), pid)
+    b = by_id(sump_r, dr.prtclid)
+    assert min_image(a, b, cfg.L) < tol * cfg.L
+    assert relerr(a[:, 3:], b[:, 3:]) < tol
+    d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_steps(gpu, prec):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    from lucky_droplets_amd.particles import MPI_particles
+    nparts = 60
+    cfg = _cfg(32, prec, nparts)
+    uk, n, coord, pid = inputs(cfg, seed=4)
+    fl = make_flow_oracle(cfg)
+    dr = make_drops_oracle(cfg, coord, pid)
+    h = 0.01
+    uk_r, n_r = uk.copy(), n.copy()
+    for _ in range(3):
+        uk_r, n_r = fl.step(0.0, h, dr, uk_r, n_r)
+    d = ForcedDNS(cfg)
+    stb = MPI_particles(None, cfg.L, cfg.N, nparts, cfg.sts, 0.05, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0,
+                        dns=d, coord=coord, prtclid=pid)
+    ukg = _t(uk, gpu, d.cdtype)
+    ng = _t(n, gpu, d.rdtype)
+    d.step(0.0, h, stb, ukg, ng, nsteps=3)
+    tol = TOL[prec]
+    assert relerr(ukg.cpu().numpy(), uk_r) < tol
+    assert relerr(ng.cpu().numpy(), n_r) < tol
+    a = by_id(stb.coord.cpu().numpy(), pid)
+    b = by_id(dr.coord, dr.prtclid)
+    assert min_image(a, b, cfg.L) < tol * cfg.L
+    assert relerr(a[:, 3:], b[:, 3:]) < tol
+    d.close()
