@@ -1,0 +1,137 @@
+#!/usr/bin/env python
+"""Benchmark: RK4 steps/s of the droplet-laden forced DNS on one MI355X.
+
+Default workload (the metric's configuration): 256^3 FP32 fields + 10^6
+droplets (FP64 positions), nu = 2.5e-3, forced shells |k| <= 2,
+E(k) ~ k^4 exp(-2 (k/3)^2) random-phase initial field (seed 0), droplet St
+lognormal (median 0.3, sigma 0.6), dt from CFL 0.5.  Inputs are generated on
+the host, copied to HBM before timing; a step is one full RK4 step (four RHS
+evaluations, droplet advance, clip_zero, reality round trip, projection,
+blow-up check) advanced on the device-resident state.
+
+Prints ONE JSON line ("Measurement").
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+METRIC = "RK steps/s @256³+1e6 droplets, FP32 vs FP64 plain; % HBM roofline"
+
+
+def algorithmic_bytes(N: int, s: int, npart: int, sp: int) -> int:
+    """Algorithmic HBM bytes of one RK4 step:
+    74 (R + C) + 4 [min(7R, 448 s Np) + R] + 4 Np 5 S_p  (see DESIGN.md)."""
+    R = N ** 3 * s
+    C = N * N * (N // 2 + 1) * 2 * s
+    return 74 * (R + C) + 4 * (min(7 * R, 448 * s * npart) + R) + 4 * npart * 5 * sp
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(cfg):
+    import numpy as np
+    from lucky_droplets_amd import ic
+    rng = np.random.default_rng(cfg.seed)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, pid = ic.initial_droplets(cfg, rng)
+    return uk, n, coord, pid, dt
+
+
+def time_gpu(cfg, uk, n, coord, dt, steps, warmup):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    dev = torch.device("cuda:0")
+    d = ForcedDNS(cfg)
+    ukg = torch.from_numpy(uk).to(dev).to(d.cdtype)
+    ng = torch.from_numpy(n).to(dev).to(d.rdtype)
+    cg = torch.from_numpy(coord).to(dev)
+    d.set_state(ukg, ng, cg)
+    del ukg, ng
+    if warmup:
+        d.advance(dt, warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    d.advance(dt, steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    lastmax, blown = d.status()
+    d.close()
+    return ms, lastmax, blown
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is a design, "
+              "see DESIGN.md 'Sharding'.", file=sys.stderr)
+        sys.exit(2)
+
+    from lucky_droplets_amd.config import preset
+    cfg = preset(args.config)
+    log(f"[bench] generating inputs for {args.config}: N={cfg.N} droplets={cfg.n_particles}")
+    t = time.time()
+    uk, n, coord, pid, dt = make_inputs(cfg)
+    log(f"[bench] inputs ready in {time.time() - t:.1f}s, dt={dt:.4e}")
+
+    log(f"[bench] FP32 timing: warmup {args.warmup}, steps {args.steps}")
+    ms32, lastmax, blown = time_gpu(cfg, uk, n, coord, dt, args.steps, args.warmup)
+    log(f"[bench] FP32 {ms32:.3f} ms/step")
+
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms32,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-phase field, seeded droplets)",
+        "config": {"workload": f"forced DNS {cfg.N}^3 + {cfg.n_particles} droplets, RK4 step",
+                   "N": cfg.N, "droplets": cfg.n_particles, "nu": cfg.nu_eff, "forced_shells": list(cfg.shell_no),
+                   "dt": dt, "precision": cfg.precision},
+    }
+    sp = 40 if cfg.precision == "f32" else 56
+    s = 4 if cfg.precision == "f32" else 8
+    B = algorithmic_bytes(cfg.N, s, cfg.n_particles, sp)
+    out["step_roofline"] = {"bytes_per_step": B, "achieved_GBs": B / (ms32 * 1e-3) / 1e9,
+                            "peak_GBs": HBM_PEAK_GBS, "frac": B / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if not args.no_fp64:
+        c64 = cfg.replace(precision="f64")
+        nsteps64 = max(2, args.steps // 4)
+        log(f"[bench] FP64 timing: steps {nsteps64}")
+        ms64, _, _ = time_gpu(c64, uk, n, coord, dt, nsteps64, 1)
+        log(f"[bench] FP64 {ms64:.3f} ms/step")
+        out["fp64"] = {"ms_per_step": ms64, "steps_per_s": 1000.0 / ms64, "fp32_speedup": ms64 / ms32}
+    out["blowup"] = bool(blown)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -46,12 +46,14 @@ __host__ __device__ constexpr double sin32(int j) { return j <= 8 ? kCos32(8 - j
 // Wavenumber of FFT index i on an N-point axis (numpy fftfreq(N, 1/N)).
 __device__ __forceinline__ int kwave(int i, int N) { return i < N / 2 ? i : i - N; }
 
-// round(|k|) for integer |k|^2 (|k| is never exactly m + 1/2 when |k|^2 is an integer).
+// round(|k|) for integer |k|^2 < 2^24: m = floor(sqrt(k2)) from sqrtf with one
+// integer correction each way (sqrtf is within one unit of the exact root
+// there), then |k| rounds up iff k2 > m^2 + m (|k| is never m + 1/2 exactly
+// when |k|^2 is an integer).
 __device__ __forceinline__ int kround(int k2) {
-  int m = (int)sqrt((double)k2);
-  while (m * m > k2) --m;
-  while ((m + 1) * (m + 1) <= k2) ++m;
-  // now m = floor(sqrt(k2)); round up when k2 > m^2 + m, i.e. |k| >= m + 1/2
+  int m = (int)sqrtf((float)k2);
+  m -= (m * m > k2) ? 1 : 0;
+  m += ((m + 1) * (m + 1) <= k2) ? 1 : 0;
   return (k2 > m * m + m) ? m + 1 : m;
 }
 

@@ -4,6 +4,7 @@
 #include "ld_engine_base.hpp"
 #include "ld_particles.hpp"
 #include <cmath>
+#include <utility>
 #include <vector>
 
 namespace ld {
@@ -75,6 +76,14 @@ struct Engine final : EngineBase {
   double *x0 = nullptr, *sx = nullptr, *kxp = nullptr;
   PT *v0 = nullptr, *sv = nullptr, *kvp = nullptr;
   double *rows_a = nullptr, *rows_b = nullptr, *wrow = nullptr;
+  // spatial sort: slot -> input row, bin keys/ranks, bin starts, double buffers
+  int64_t *rid = nullptr, *rid2 = nullptr;
+  int *pkey = nullptr, *prank = nullptr, *hist = nullptr, *bsum = nullptr;
+  double* x0b = nullptr;
+  PT* v0b = nullptr;
+  bool sort_enabled = true;
+  static constexpr int NB = Bins<N>::NB;
+  static constexpr int NBS = (NB + 4095) / 4096;
   DropPhys dp;
 
   explicit Engine(const ld_params& prm) : p(prm) {}
@@ -131,7 +140,11 @@ struct Engine final : EngineBase {
       LD_TRY(alloc(x0, 3 * cap)); LD_TRY(alloc(sx, 3 * cap)); LD_TRY(alloc(kxp, 3 * cap));
       LD_TRY(alloc(v0, 4 * cap)); LD_TRY(alloc(sv, 4 * cap)); LD_TRY(alloc(kvp, 4 * cap));
       LD_TRY(alloc(rows_a, 7 * cap)); LD_TRY(alloc(rows_b, 7 * cap)); LD_TRY(alloc(wrow, cap));
+      LD_TRY(alloc(rid, cap)); LD_TRY(alloc(rid2, cap)); LD_TRY(alloc(pkey, cap)); LD_TRY(alloc(prank, cap));
+      LD_TRY(alloc(hist, NB)); LD_TRY(alloc(bsum, NBS));
+      LD_TRY(alloc(x0b, 3 * cap)); LD_TRY(alloc(v0b, 4 * cap));
     }
+    static_assert(NBS <= 4096, "bin-start scan supports at most 4096 partial sums");
     dp.L = p.L;
     dp.dx = p.L / N;                       // np.linspace(0, L, N, endpoint=False) step
     dp.pi_2dx = M_PI / (2.0 * dp.dx);
@@ -277,7 +290,9 @@ struct Engine final : EngineBase {
       ps.x0 = x0; ps.v0 = v0; ps.sx = sx; ps.sv = sv; ps.kx = kxp; ps.kv = kvp;
       ps.u = u; ps.v = v; ps.n = ns; ps.clip = clip_stage; ps.fc = fc;
       ps.ah = a_coef[s] * h; ps.wh = wh; ps.P = dp;
-      hipLaunchKernelGGL((k_particles_stage<T, PT, N>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, ps);
+      // 4 lanes per droplet; block count a multiple of 8 for the XCD mapping
+      const unsigned nb = (unsigned)(((4 * np + 255) / 256 + 7) / 8 * 8);
+      hipLaunchKernelGGL((k_particles_stage<T, PT, N>), dim3(nb), dim3(256), 0, st, ps);
       LD_LAUNCHED();
     }
     LD_TRY(density_rhs(fc));
@@ -326,13 +341,15 @@ struct Engine final : EngineBase {
       hipLaunchKernelGGL((k_rows_to_soa<PT>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, coord, (int)np,
                          (int)cap, x0, v0);
       LD_LAUNCHED();
+      hipLaunchKernelGGL(k_iota64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, rid, (int)np);
+      LD_LAUNCHED();
     }
     return LD_OK;
   }
   int get_rows(double* coord) {
     if (np > 0) {
       hipLaunchKernelGGL((k_soa_to_rows<PT>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
-                         (const double*)x0, (const PT*)v0, (int)np, (int)cap, coord);
+                         (const double*)x0, (const PT*)v0, (const int64_t*)rid, (int)np, (int)cap, coord);
       LD_LAUNCHED();
     }
     return LD_OK;
@@ -348,8 +365,33 @@ struct Engine final : EngineBase {
     if (coord) LD_TRY(get_rows(coord));
     return LD_OK;
   }
+  // Counting sort of the droplet slots by spatial bin of the base position.
+  int sort_droplets() {
+    if (np < 2 || !sort_enabled) return LD_OK;
+    const unsigned pb = (unsigned)((np + 255) / 256);
+    LD_TRY(fill<int>(hist, NB, 0));
+    hipLaunchKernelGGL((k_bin_count<N>), dim3(pb), dim3(256), 0, st, (const double*)x0, (int)np, (int)cap, p.L,
+                       dp.dx, pkey, prank, hist);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_local, dim3(NBS), dim3(1024), 0, st, hist, NB, bsum);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, st, bsum, NBS);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_add, dim3((NB + 255) / 256), dim3(256), 0, st, hist, NB, (const int*)bsum);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL((k_bin_scatter<PT>), dim3(pb), dim3(256), 0, st, (int)np, (int)cap, (const int*)pkey,
+                       (const int*)prank, (const int*)hist, (const double*)x0, (const PT*)v0, (const int64_t*)rid,
+                       x0b, v0b, rid2);
+    LD_LAUNCHED();
+    std::swap(x0, x0b);
+    std::swap(v0, v0b);
+    std::swap(rid, rid2);
+    return LD_OK;
+  }
+
   int advance(double h, int nsteps) override {
     for (int i = 0; i < nsteps; ++i) {
+      LD_TRY(sort_droplets());
       LD_TRY(rk4_stages(h));
       LD_TRY(epilogue());
     }

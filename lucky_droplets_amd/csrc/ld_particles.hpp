@@ -172,56 +172,119 @@ template <typename T, typename PT> struct PStage {
 // (forced-dns-sm.py:347, 352, 357), wrap (particles.py:269), gather of
 // [u, v, n] (:826), growth rate and deposit (:828-829), RHS (:831-835) and
 // sump accumulation (forced-dns-sm.py:342-358).
+//
+// Four lanes per droplet: lane c owns z-node c of the 4x4x4 stencil, so one
+// wavefront load or atomic touches 16 droplets x 4 consecutive z cells (one
+// 16-byte segment per droplet).  Partial sums are combined across the four
+// lanes with xor shuffles.
 template <typename T, typename PT, int N>
 __global__ void __launch_bounds__(256)
 k_particles_stage(PStage<T, PT> A) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.np) return;
+  // XCD-aware mapping: the 8 XCDs take consecutive blocks round-robin; give
+  // each XCD one contiguous range of (spatially sorted) droplets so its L2
+  // sees one region of the fields.
+  const int nb = gridDim.x, per = nb / 8;
+  const int lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int gt = lb * blockDim.x + threadIdx.x;
+  const int p = gt >> 2, c = gt & 3;
+  const bool live = p < A.np;
+  const int pp = live ? p : 0;
   constexpr size_t R = (size_t)N * N * N;
   const int cap = A.cap;
   double xs[3], vp[3], m;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double x = A.x0[c * cap + p];
-    double vv = (double)A.v0[c * cap + p];
+  for (int k = 0; k < 3; ++k) {
+    double x = A.x0[k * cap + pp];
+    double vv = (double)A.v0[k * cap + pp];
     if (A.stage > 0) {
-      x = x + A.ah * A.kx[c * cap + p];
-      vv = (double)(PT)(A.v0[c * cap + p] + (PT)A.ah * A.kv[c * cap + p]);
+      x = x + A.ah * A.kx[k * cap + pp];
+      vv = (double)(PT)(A.v0[k * cap + pp] + (PT)A.ah * A.kv[k * cap + pp]);
     }
-    xs[c] = x; vp[c] = vv;
+    xs[k] = x; vp[k] = vv;
   }
-  m = (double)A.v0[3 * cap + p];
-  if (A.stage > 0) m = (double)(PT)(A.v0[3 * cap + p] + (PT)A.ah * A.kv[3 * cap + p]);
-  double xw[3];
+  m = (double)A.v0[3 * cap + pp];
+  if (A.stage > 0) m = (double)(PT)(A.v0[3 * cap + pp] + (PT)A.ah * A.kv[3 * cap + pp]);
+  const double L = A.P.L, dx = A.P.dx, pi_2dx = A.P.pi_2dx;
+  const double xw = wrapL(xs[0], L), yw = wrapL(xs[1], L), zw = wrapL(xs[2], L);
+  const int i0 = (int)floor(xw / dx) - 1, j0 = (int)floor(yw / dx) - 1, k0 = (int)floor(zw / dx) - 1;
+  double wy[4];
+  int iy[4];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) xw[c] = wrapL(xs[c], A.P.L);
-  Stencil<N> s;
-  s.set(xw, A.P.dx, A.P.pi_2dx);
+  for (int b = 0; b < 4; ++b) { wy[b] = cosW(yw, j0 + b, dx, pi_2dx); iy[b] = wrapIdx<N>(j0 + b); }
+  const double wz = cosW(zw, k0 + c, dx, pi_2dx);
+  const int iz = wrapIdx<N>(k0 + c);
+  // gather: this lane's z node, all (x, y) nodes
   double I[7];
-  gatherN<T, N, 3>(s, A.u, R, I);
-  gatherN<T, N, 3>(s, A.v, R, I + 3);
-  I[6] = gatherDensity<T, N>(s, A.n, A.clip);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) I[k] = 0.0;
+  const bool doclip = A.clip[2] != 0.0;
+  if (live) {
+#pragma unroll 1
+    for (int a = 0; a < 4; ++a) {
+      const double wx = cosW(xw, i0 + a, dx, pi_2dx);
+      const size_t plane = (size_t)wrapIdx<N>(i0 + a) * N;
+      double t[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) t[k] = 0.0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const size_t o = (plane + iy[b]) * N + iz;
+        const double w = wy[b];
+        t[0] += (double)A.u[o] * w;
+        t[1] += (double)A.u[R + o] * w;
+        t[2] += (double)A.u[2 * R + o] * w;
+        t[3] += (double)A.v[o] * w;
+        t[4] += (double)A.v[R + o] * w;
+        t[5] += (double)A.v[2 * R + o] * w;
+        double nv = (double)A.n[o];
+        if (doclip && nv < 0.0) nv = 0.0;
+        t[6] += nv * w;
+      }
+#pragma unroll
+      for (int k = 0; k < 7; ++k) I[k] += t[k] * wx;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    double v = I[k] * wz;
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    I[k] = v;
+  }
+  if (!live) return;
+  if (doclip) I[6] = (I[6] * A.clip[0]) / A.clip[1];
   double rhs[7], W;
   dropletRhs(A.P, vp, m, I, rhs, W);
-  deposit<T, N>(s, W, A.P.inv_vol, A.fc);
+  // deposit of W with the same kernel / dx^3 (particles.py:23-35, 529-575)
+  {
+    const double wzv = W * wz * A.P.inv_vol;
+#pragma unroll 1
+    for (int a = 0; a < 4; ++a) {
+      const double wa = wzv * cosW(xw, i0 + a, dx, pi_2dx);
+      const size_t plane = (size_t)wrapIdx<N>(i0 + a) * N;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) unsafeAtomicAdd(&A.fc[(plane + iy[b]) * N + iz], (T)(wa * wy[b]));
+    }
+  }
+  if (c != 0) return;
   // sump (forced-dns-sm.py:337, 342-358); the last stage writes the new state
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double prev = A.stage == 0 ? A.x0[c * cap + p] : A.sx[c * cap + p];
-    const double nv = prev + A.wh * rhs[c];
-    if (A.stage == 3) A.x0[c * cap + p] = nv; else A.sx[c * cap + p] = nv;
+  for (int k = 0; k < 3; ++k) {
+    const double prev = A.stage == 0 ? A.x0[k * cap + p] : A.sx[k * cap + p];
+    const double nv = prev + A.wh * rhs[k];
+    if (A.stage == 3) A.x0[k * cap + p] = nv; else A.sx[k * cap + p] = nv;
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const PT prev = A.stage == 0 ? A.v0[c * cap + p] : A.sv[c * cap + p];
-    const PT nv = prev + (PT)A.wh * (PT)rhs[3 + c];
-    if (A.stage == 3) A.v0[c * cap + p] = nv; else A.sv[c * cap + p] = nv;
+  for (int k = 0; k < 4; ++k) {
+    const PT prev = A.stage == 0 ? A.v0[k * cap + p] : A.sv[k * cap + p];
+    const PT nv = prev + (PT)A.wh * (PT)rhs[3 + k];
+    if (A.stage == 3) A.v0[k * cap + p] = nv; else A.sv[k * cap + p] = nv;
   }
   if (A.stage < 3) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) A.kx[c * cap + p] = rhs[c];
+    for (int k = 0; k < 3; ++k) A.kx[k * cap + p] = rhs[k];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) A.kv[c * cap + p] = (PT)rhs[3 + c];
+    for (int k = 0; k < 4; ++k) A.kv[k * cap + p] = (PT)rhs[3 + k];
   }
 }
 
@@ -293,18 +356,141 @@ __global__ void k_rows_to_soa(const double* __restrict__ rows, int np, int cap, 
   for (int c = 0; c < 3; ++c) pos[c * cap + p] = rows[(size_t)p * 7 + c];
   for (int c = 0; c < 4; ++c) vm[c * cap + p] = (PT)rows[(size_t)p * 7 + 3 + c];
 }
+// rid[p]: the input row of engine slot p (slots are permuted by the spatial sort).
 template <typename PT>
-__global__ void k_soa_to_rows(const double* __restrict__ pos, const PT* __restrict__ vm, int np, int cap, double* rows) {
+__global__ void k_soa_to_rows(const double* __restrict__ pos, const PT* __restrict__ vm, const int64_t* __restrict__ rid,
+                              int np, int cap, double* rows) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  for (int c = 0; c < 3; ++c) rows[(size_t)p * 7 + c] = pos[c * cap + p];
-  for (int c = 0; c < 4; ++c) rows[(size_t)p * 7 + 3 + c] = (double)vm[c * cap + p];
+  const size_t r = (size_t)rid[p];
+  for (int c = 0; c < 3; ++c) rows[r * 7 + c] = pos[c * cap + p];
+  for (int c = 0; c < 4; ++c) rows[r * 7 + 3 + c] = (double)vm[c * cap + p];
 }
 
 // Stage coordinate rows X + a*kp (forced-dns-sm.py:347) for the API path.
 static __global__ void k_axpy_rows(const double* __restrict__ x, const double* __restrict__ k, double a, int n, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = x[i] + a * k[i];
+}
+
+// ------------------------------------------------------------ spatial sort
+// Droplets are reordered by spatial bin once per step (counting sort), so
+// that lanes of a wavefront gather from and deposit into the same cache
+// lines.  Bins are cubes of BS = max(2, N/128) grid cells.
+template <int N> struct Bins {
+  static constexpr int BS = (N / 128 > 2) ? N / 128 : 2;
+  static constexpr int B = N / BS;
+  static constexpr int NB = B * B * B;
+};
+
+template <int N>
+__global__ void __launch_bounds__(256)
+k_bin_count(const double* __restrict__ pos, int np, int cap, double L, double dx, int* __restrict__ key,
+            int* __restrict__ rank, int* __restrict__ hist) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  int b[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int ci = (int)(wrapL(pos[c * cap + p], L) / dx);
+    ci = ci < 0 ? 0 : (ci > N - 1 ? N - 1 : ci);
+    b[c] = ci / Bins<N>::BS;
+  }
+  const int k = (b[0] * Bins<N>::B + b[1]) * Bins<N>::B + b[2];
+  key[p] = k;
+  rank[p] = atomicAdd(&hist[k], 1);
+}
+
+// Exclusive scan of 4096 ints per block (1024 threads x 4); block totals to bsum.
+__device__ __forceinline__ int blockExclusiveScan1024(int v, int* sh, int& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    int w = lane < 16 ? sh[lane] : 0;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const int y = __shfl_up(w, d, 64);
+      if (lane >= d) w += y;
+    }
+    if (lane < 16) sh[16 + lane] = w;   // inclusive wave prefix
+  }
+  __syncthreads();
+  const int wpre = wid > 0 ? sh[16 + wid - 1] : 0;
+  total = sh[16 + 15];
+  __syncthreads();
+  return wpre + x - v;
+}
+
+static __global__ void __launch_bounds__(1024)
+k_scan_local(int* __restrict__ data, int n, int* __restrict__ bsum) {
+  __shared__ int sh[32];
+  const int base = blockIdx.x * 4096 + threadIdx.x * 4;
+  int v[4], s = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { v[i] = (base + i < n) ? data[base + i] : 0; s += v[i]; }
+  int total;
+  int pre = blockExclusiveScan1024(s, sh, total);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (base + i < n) data[base + i] = pre;
+    pre += v[i];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+static __global__ void __launch_bounds__(1024)
+k_scan_bsums(int* __restrict__ bsum, int nb) {
+  __shared__ int sh[32];
+  const int base = threadIdx.x * 4;
+  int v[4], s = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { v[i] = (base + i < nb) ? bsum[base + i] : 0; s += v[i]; }
+  int total;
+  int pre = blockExclusiveScan1024(s, sh, total);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (base + i < nb) bsum[base + i] = pre;
+    pre += v[i];
+  }
+}
+
+static __global__ void __launch_bounds__(256)
+k_scan_add(int* __restrict__ data, int n, const int* __restrict__ bsum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) data[i] += bsum[i / 4096];
+}
+
+// Scatter the droplet state to its sorted slot.
+template <typename PT>
+__global__ void __launch_bounds__(256)
+k_bin_scatter(int np, int cap, const int* __restrict__ key, const int* __restrict__ rank,
+              const int* __restrict__ start, const double* __restrict__ pos, const PT* __restrict__ vm,
+              const int64_t* __restrict__ id, double* __restrict__ pos2, PT* __restrict__ vm2,
+              int64_t* __restrict__ id2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int d = start[key[p]] + rank[p];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) pos2[c * cap + d] = pos[c * cap + p];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) vm2[c * cap + d] = vm[c * cap + p];
+  id2[d] = id[p];
+}
+
+template <typename X>
+__global__ void k_copy(const X* __restrict__ a, X* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+static __global__ void k_iota64(int64_t* __restrict__ id, int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) id[p] = p;
 }
 
 }  // namespace ld
