@@ -50,7 +50,9 @@ def make_inputs(cfg):
     return uk, n, coord, pid, dt
 
 
-def time_gpu(cfg, uk, n, coord, dt, steps, warmup):
+def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None):
+    """ms per step over `steps` timed steps (after `warmup`), plus the summed
+    HIP-event duration and launch count of `profile_kernel` inside the timed region."""
     import torch
     from lucky_droplets_amd.dns import ForcedDNS
     dev = torch.device("cuda:0")
@@ -63,6 +65,8 @@ def time_gpu(cfg, uk, n, coord, dt, steps, warmup):
     if warmup:
         d.advance(dt, warmup)
     torch.cuda.synchronize()
+    if profile_kernel:
+        d.profile(profile_kernel, True)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -70,9 +74,50 @@ def time_gpu(cfg, uk, n, coord, dt, steps, warmup):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
+    kern = d.profile_read() if profile_kernel else (0.0, 0)
+    if profile_kernel:
+        d.profile(profile_kernel, False)
     lastmax, blown = d.status()
     d.close()
-    return ms, lastmax, blown
+    return ms, kern, lastmax, blown
+
+
+def droplet_launch_bytes(N, s, npart, sp):
+    """Algorithmic bytes of one droplet-stage launch (one quarter of the
+    per-step droplet terms): min(7R, 448 s Np) + R + 5 Np S_p."""
+    R = N ** 3 * s
+    return min(7 * R, 448 * s * npart) + R + 5 * npart * sp
+
+
+def cpu_baseline(uk, n, coord, cfg, dt):
+    """The CPU FP64 oracle (numpy + scipy.fft) on the same inputs, bounded
+    sample: one full_RHS evaluation and one step epilogue, scaled to steps/s
+    as 1 / (4 t_rhs + t_epilogue)."""
+    import oracle
+    import oracle.flow as oflow
+    workers = min(16, os.cpu_count() or 1)
+    oflow.WORKERS = workers
+    fl = oracle.Flow(cfg.N, cfg.nu_eff, cfg.shell_no, cfg.f0, cfg.tps, cfg.g, cfg.isforcing,
+                     1.0 if cfg.explicit else 0.0, cfg.lp, cfg.L)
+    dr = oracle.Droplets(cfg.L, cfg.N, coord.copy(), cfg.sts, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0)
+    t = time.time()
+    fl.full_RHS(0.0, uk, n.copy(), dr.coord.copy(), dr.coord.copy(), dr)
+    t_rhs = time.time() - t
+    t = time.time()
+    fl.clip_zero(n.copy())
+    fl.reality_and_projection(uk)
+    t_epi = time.time() - t
+    step_s = 4 * t_rhs + t_epi
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": workers, "kind": "port",
+            "sample": f"one full_RHS ({t_rhs:.1f} s) + one epilogue ({t_epi:.1f} s) of the FP64 oracle at "
+                      f"{cfg.N}^3 + {cfg.n_particles} droplets, scipy.fft workers={workers}, "
+                      f"step = 4 RHS + epilogue; host: {model}"}
 
 
 def main():
@@ -97,8 +142,9 @@ def main():
     log(f"[bench] inputs ready in {time.time() - t:.1f}s, dt={dt:.4e}")
 
     log(f"[bench] FP32 timing: warmup {args.warmup}, steps {args.steps}")
-    ms32, lastmax, blown = time_gpu(cfg, uk, n, coord, dt, args.steps, args.warmup)
-    log(f"[bench] FP32 {ms32:.3f} ms/step")
+    ms32, (kms, klaunch), lastmax, blown = time_gpu(cfg, uk, n, coord, dt, args.steps, args.warmup,
+                                                    profile_kernel="droplets" if cfg.n_particles else None)
+    log(f"[bench] FP32 {ms32:.3f} ms/step; droplet kernel {kms / max(klaunch, 1):.3f} ms x {klaunch}")
 
     out = {
         "metric": METRIC,
@@ -122,14 +168,31 @@ def main():
     B = algorithmic_bytes(cfg.N, s, cfg.n_particles, sp)
     out["step_roofline"] = {"bytes_per_step": B, "achieved_GBs": B / (ms32 * 1e-3) / 1e9,
                             "peak_GBs": HBM_PEAK_GBS, "frac": B / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if klaunch:
+        kb = droplet_launch_bytes(cfg.N, s, cfg.n_particles, sp)
+        avg = kms / klaunch
+        ach = kb / (avg * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic_droplets_256.json")
+        if os.path.exists(tpath) and args.config == "c2":
+            with open(tpath) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        out["roofline"] = {"bound": "hbm", "kernel": "k_particles_stage (droplet RK stage)",
+                           "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                           "traffic": traffic, "algorithmic_bytes_per_launch": kb, "avg_launch_ms": avg,
+                           "launches": klaunch}
     if not args.no_fp64:
         c64 = cfg.replace(precision="f64")
         nsteps64 = max(2, args.steps // 4)
         log(f"[bench] FP64 timing: steps {nsteps64}")
-        ms64, _, _ = time_gpu(c64, uk, n, coord, dt, nsteps64, 1)
+        ms64, _, _, _ = time_gpu(c64, uk, n, coord, dt, nsteps64, 1)
         log(f"[bench] FP64 {ms64:.3f} ms/step")
         out["fp64"] = {"ms_per_step": ms64, "steps_per_s": 1000.0 / ms64, "fp32_speedup": ms64 / ms32}
     out["blowup"] = bool(blown)
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle, bounded sample)")
+        out["cpu_baseline"] = cpu_baseline(uk, n, coord, cfg, dt)
+        log(f"[bench] CPU {out['cpu_baseline']['value']:.4f} steps/s")
     print(json.dumps(out), flush=True)
 
 

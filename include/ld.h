@@ -113,6 +113,24 @@ int ld_advance(ld_ctx* ctx, double h, int32_t nsteps);
 /* Last step's max Re(uk) (host double) and sticky blow-up flag. */
 int ld_status(ld_ctx* ctx, double* last_max, int32_t* blown_up);
 
+/* ---------------------------------------------------------- measurement */
+/* Kernel classes that ld_profile can time (HIP events recorded on the stream
+ * each launch runs on).  No counterpart in the reference, which only prints
+ * wall time per step (forced-dns-sm.py:565-577). */
+#define LD_K_DROPLETS 0   /* fused droplet RK stage: gather, RHS, deposit */
+#define LD_K_XRHS 1       /* x pass: nonlinear term, forcing, projection, RK update, v_hat */
+#define LD_K_XUW 2        /* x pass: u_hat, i k x u_hat */
+#define LD_K_YPASS 3      /* y line passes (all) */
+#define LD_K_ZCROSS 4     /* z pass: u, omega, u x omega */
+#define LD_K_ZVEL 5       /* z pass: v, n v */
+#define LD_K_XKN 6        /* x pass: density flux */
+#define LD_K_ZKN 7        /* z pass: kn, RK update of n */
+/* Start timing every launch of kernel class `kernel` (or stop, enable = 0). */
+int ld_profile(ld_ctx* ctx, int32_t kernel, int32_t enable);
+/* Synchronise, then report the summed duration (ms) and launch count since
+ * ld_profile was enabled; resets the counters. */
+int ld_profile_read(ld_ctx* ctx, double* total_ms, int64_t* launches);
+
 /* ------------------------------------------------------------ droplets */
 /* MPI_particles.interp_cosine(coord, u): 4x4x4 cosine-kernel gather of ncomp
  * physical fields at the rows' positions -> out (np, ncomp)   particles.py:435-511 */

@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libld.so"
 
 LD_OK, LD_ERR_ARG, LD_ERR_HIP, LD_ERR_BLOWUP, LD_ERR_CAPACITY = 0, 1, 2, 3, 4
 LD_F32, LD_F64 = 0, 1
+KERNELS = {"droplets": 0, "x_rhs": 1, "x_uw": 2, "y_pass": 3, "z_cross": 4, "z_vel": 5, "x_kn": 6, "z_kn": 7}
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 
@@ -73,6 +74,8 @@ _SIGS = {
     "ld_state_get": [_VP, _VP, _VP, _VP],
     "ld_advance": [_VP, ctypes.c_double, ctypes.c_int32],
     "ld_status": [_VP, _c_double_p, ctypes.POINTER(ctypes.c_int32)],
+    "ld_profile": [_VP, ctypes.c_int32, ctypes.c_int32],
+    "ld_profile_read": [_VP, _c_double_p, ctypes.POINTER(ctypes.c_int64)],
     "ld_interp_cosine": [_VP, _VP, ctypes.c_int64, _VP, ctypes.c_int32, _VP],
     "ld_exterp_cosine_scalar": [_VP, _VP, ctypes.c_int64, _VP, _VP],
     "ld_prhs": [_VP, _VP, ctypes.c_int64, _VP, _VP, _VP, _VP, _VP, _VP, _VP],

@@ -179,6 +179,17 @@ int ld_status(ld_ctx* ctx, double* last_max, int32_t* blown_up) {
   return ctx->eng->status(last_max, blown_up);
 }
 
+int ld_profile(ld_ctx* ctx, int32_t kernel, int32_t enable) {
+  LD_CHECK_CTX(ctx);
+  if (kernel < 0 || kernel > LD_K_ZKN) { set_error("unknown kernel class %d", kernel); return LD_ERR_ARG; }
+  return ctx->eng->profile(kernel, enable);
+}
+
+int ld_profile_read(ld_ctx* ctx, double* total_ms, int64_t* launches) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(total_ms); LD_NONNULL(launches);
+  return ctx->eng->profile_read(total_ms, launches);
+}
+
 int ld_interp_cosine(ld_ctx* ctx, const double* coord, int64_t np, const void* fields, int32_t ncomp, double* out) {
   LD_CHECK_CTX(ctx);
   if (np < 0 || ncomp < 1) { set_error("np must be >= 0 and ncomp >= 1"); return LD_ERR_ARG; }

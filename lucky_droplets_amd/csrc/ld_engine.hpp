@@ -66,9 +66,12 @@ struct Engine final : EngineBase {
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
   C<T>* tw = nullptr;
-  C<T>*uk = nullptr, *us = nullptr, *acc = nullptr, *w6 = nullptr;
-  T *u = nullptr, *v = nullptr, *n_state = nullptr, *n_stage = nullptr, *nacc = nullptr, *fc = nullptr;
-  double *partials = nullptr, *clip_stage = nullptr, *clip_end = nullptr, *clip_none = nullptr;
+  hipStream_t st2 = nullptr;        // density/droplet stream (created here)
+  hipEvent_t evA = nullptr, evB = nullptr;
+  C<T>*uk = nullptr, *us = nullptr, *acc = nullptr, *w6 = nullptr, *wn = nullptr;
+  T* ubuf[2] = {nullptr, nullptr};
+  T *v = nullptr, *n_state = nullptr, *n_stage = nullptr, *nacc = nullptr, *fc = nullptr;
+  double *partials = nullptr, *partials2 = nullptr, *clip_stage = nullptr, *clip_end = nullptr, *clip_none = nullptr;
   double *maxpart = nullptr, *ek = nullptr, *fac = nullptr, *lastmax = nullptr;
   int *flag = nullptr, *shells = nullptr;
   int smax = 0;
@@ -85,9 +88,18 @@ struct Engine final : EngineBase {
   static constexpr int NB = Bins<N>::NB;
   static constexpr int NBS = (NB + 4095) / 4096;
   DropPhys dp;
+  // kernel timing (ld_profile): event pairs around launches of one kernel class
+  int prof_id = -1;
+  std::vector<hipEvent_t> prof_ev;
+  size_t prof_used = 0;
+  static constexpr size_t PROF_MAX = 8192;
 
   explicit Engine(const ld_params& prm) : p(prm) {}
   ~Engine() override {
+    if (st2) { (void)hipStreamSynchronize(st2); (void)hipStreamDestroy(st2); }
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
+    if (evA) (void)hipEventDestroy(evA);
+    if (evB) (void)hipEventDestroy(evB);
     for (void* a : allocs) (void)hipFree(a);
   }
 
@@ -120,11 +132,16 @@ struct Engine final : EngineBase {
     }
     LD_TRY(alloc(tw, N));
     LD_HIP(hipMemcpy(tw, h.data(), N * sizeof(C<T>), hipMemcpyHostToDevice));
+    LD_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));   // default priority
+    LD_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
+    LD_HIP(hipEventCreateWithFlags(&evB, hipEventDisableTiming));
     LD_TRY(alloc(uk, 3 * CF)); LD_TRY(alloc(us, 3 * CF)); LD_TRY(alloc(acc, 3 * CF)); LD_TRY(alloc(w6, 6 * CF));
-    LD_TRY(alloc(u, 3 * RF)); LD_TRY(alloc(v, 3 * RF));
+    LD_TRY(alloc(wn, 4 * CF));
+    LD_TRY(alloc(ubuf[0], 3 * RF)); LD_TRY(alloc(ubuf[1], 3 * RF)); LD_TRY(alloc(v, 3 * RF));
     LD_TRY(alloc(n_state, RF)); LD_TRY(alloc(n_stage, RF)); LD_TRY(alloc(nacc, RF)); LD_TRY(alloc(fc, RF));
     const int npart = NBLK_Z1 > NBLK_SUM ? NBLK_Z1 : NBLK_SUM;
     LD_TRY(alloc(partials, 2 * (size_t)npart));
+    LD_TRY(alloc(partials2, 2 * (size_t)npart));
     LD_TRY(alloc(clip_stage, 4)); LD_TRY(alloc(clip_end, 4)); LD_TRY(alloc(clip_none, 4));
     LD_TRY(alloc(maxpart, NBLK_X)); LD_TRY(alloc(ek, Nf)); LD_TRY(alloc(fac, Nf)); LD_TRY(alloc(lastmax, 1));
     LD_TRY(alloc(flag, 1)); LD_TRY(alloc(shells, 8));
@@ -156,37 +173,81 @@ struct Engine final : EngineBase {
   void set_stream(hipStream_t s) override { st = s; }
 
   // ------------------------------------------------------------ launch helpers
-  template <int AXIS, int DIR> int pass(const C<T>* in, C<T>* out, int nf, T scale) {
-    hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(NBLK_X, nf), dim3(M * N1), 0, st, in, out, CF,
+  template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale) {
+    const int kid = AXIS == 1 ? LD_K_YPASS : -2;
+    LD_TRY(prof_mark(kid, s));
+    hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(NBLK_X, nf), dim3(M * N1), 0, s, in, out, CF,
                        (const C<T>*)tw, scale);
+    LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
   }
-  template <int P, int FIN, int FOUT, class Op> int zrun(const Op& op) {
-    hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(N * N / P), dim3(ZThreads<N, P, FIN, FOUT>::v), 0, st, op,
+  template <int P, int FIN, int FOUT, class Op> int zrun(hipStream_t s, const Op& op, int kid = -2) {
+    LD_TRY(prof_mark(kid, s));
+    hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(N * N / P), dim3(ZThreads<N, P, FIN, FOUT>::v), 0, s, op,
                        (const C<T>*)tw);
+    LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
   }
-  template <class X> int fill(X* x, size_t n, X val) {
-    hipLaunchKernelGGL((k_fill<X>), dim3(1024), dim3(256), 0, st, x, n, val);
+  template <class X> int fill(hipStream_t s, X* x, size_t n, X val) {
+    hipLaunchKernelGGL((k_fill<X>), dim3(1024), dim3(256), 0, s, x, n, val);
     LD_LAUNCHED();
     return LD_OK;
   }
-  int shell(const C<T>* uin, double fscale) {
+  int shell(hipStream_t s, const C<T>* uin, double fscale) {
     const double norm0 = 1.0 / ((double)N * N * N * N * N * N / std::pow(2.0 * M_PI, 3.0));
     if (p.nshells == 0) return LD_OK;
-    hipLaunchKernelGGL((k_shell<T, N>), dim3(1), dim3(256), 0, st, uin, CF, smax, (const int*)shells, p.nshells,
+    hipLaunchKernelGGL((k_shell<T, N>), dim3(1), dim3(256), 0, s, uin, CF, smax, (const int*)shells, p.nshells,
                        p.f0, fscale, norm0, ek, fac);
     LD_LAUNCHED();
     return LD_OK;
   }
-  int clip_sums(const T* x, double* clip) {
-    hipLaunchKernelGGL((k_sum2<T>), dim3(NBLK_SUM), dim3(256), 0, st, x, RF, partials);
+  int clip_sums(hipStream_t s, const T* x, double* clip) {
+    hipLaunchKernelGGL((k_sum2<T>), dim3(NBLK_SUM), dim3(256), 0, s, x, RF, partials);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_clip_final, dim3(1), dim3(256), 0, st, (const double*)partials, NBLK_SUM,
+    hipLaunchKernelGGL(k_clip_final, dim3(1), dim3(256), 0, s, (const double*)partials, NBLK_SUM,
                        1.0 / ((double)N * N * N), clip);
     LD_LAUNCHED();
+    return LD_OK;
+  }
+  // Event pair around one launch of kernel class `id` (when it is profiled).
+  int prof_mark(int id, hipStream_t s) {
+    if (id != prof_id) return LD_OK;
+    if (prof_used >= prof_ev.size()) {
+      if (prof_ev.size() >= PROF_MAX) return LD_OK;   // pool exhausted: stop recording
+      hipEvent_t e;
+      LD_HIP(hipEventCreate(&e));
+      prof_ev.push_back(e);
+    }
+    LD_HIP(hipEventRecord(prof_ev[prof_used++], s));
+    return LD_OK;
+  }
+  int profile(int kernel, int enable) override {
+    prof_id = enable ? kernel : -1;
+    prof_used = 0;
+    return LD_OK;
+  }
+  int profile_read(double* total_ms, int64_t* launches) override {
+    LD_HIP(hipDeviceSynchronize());
+    double tot = 0.0;
+    const size_t pairs = prof_used / 2;
+    for (size_t i = 0; i < pairs; ++i) {
+      float ms = 0.f;
+      LD_HIP(hipEventElapsedTime(&ms, prof_ev[2 * i], prof_ev[2 * i + 1]));
+      tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int64_t)pairs;
+    prof_used = 0;
+    return LD_OK;
+  }
+
+  // make `waiter` wait for all work so far on `signaller`
+  int join(hipStream_t waiter, hipStream_t signaller, hipEvent_t ev) {
+    if (waiter == signaller) return LD_OK;
+    LD_HIP(hipEventRecord(ev, signaller));
+    LD_HIP(hipStreamWaitEvent(waiter, ev, 0));
     return LD_OK;
   }
   static T inv() { return (T)(1.0 / N); }
@@ -197,10 +258,10 @@ struct Engine final : EngineBase {
     C<T>* dst = (C<T>*)fuout;
     for (int f = 0; f < nf; ++f) {
       OpZFwd<T, N, P1, 1> op{src + f * RF, dst + f * CF, RF, CF};
-      LD_TRY((zrun<P1, 0, 1>(op)));
+      LD_TRY((zrun<P1, 0, 1>(st, op)));
     }
-    LD_TRY((pass<1, -1>(dst, dst, nf, (T)1)));
-    LD_TRY((pass<0, -1>(dst, dst, nf, (T)1)));
+    LD_TRY((pass<1, -1>(st, dst, dst, nf, (T)1)));
+    LD_TRY((pass<0, -1>(st, dst, dst, nf, (T)1)));
     return LD_OK;
   }
   int irfft(const void* fuin, void* uout, int nf) override {
@@ -208,18 +269,18 @@ struct Engine final : EngineBase {
     T* dst = (T*)uout;
     for (int f0 = 0; f0 < nf; f0 += 6) {
       const int k = nf - f0 < 6 ? nf - f0 : 6;
-      LD_TRY((pass<0, +1>(src + f0 * CF, w6, k, inv())));
-      LD_TRY((pass<1, +1>(w6, w6, k, inv())));
+      LD_TRY((pass<0, +1>(st, src + f0 * CF, w6, k, inv())));
+      LD_TRY((pass<1, +1>(st, w6, w6, k, inv())));
       for (int f = 0; f < k; ++f) {
         OpZInv<T, N, P1, 1> op{w6 + f * CF, dst + (f0 + f) * RF, CF, RF, inv()};
-        LD_TRY((zrun<P1, 1, 0>(op)));
+        LD_TRY((zrun<P1, 1, 0>(st, op)));
       }
     }
     return LD_OK;
   }
 
   int forcing(const void* ukin, void* fkout, double* ek_out) override {
-    LD_TRY(shell((const C<T>*)ukin, p.isforcing));
+    LD_TRY(shell(st, (const C<T>*)ukin, p.isforcing));
     const size_t nb = (CF + 255) / 256;
     hipLaunchKernelGGL((k_forcing_apply<T, N>), dim3(nb), dim3(256), 0, st, (const C<T>*)ukin, (C<T>*)fkout, CF,
                        (const double*)fac);
@@ -229,101 +290,130 @@ struct Engine final : EngineBase {
   }
 
   int clip_zero(void* x, void* out) override {
-    LD_TRY(clip_sums((const T*)x, clip_end));
+    LD_TRY(clip_sums(st, (const T*)x, clip_end));
     hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st, (T*)x, (T*)out, RF, (const double*)clip_end, 1);
     LD_LAUNCHED();
     return LD_OK;
   }
 
   // ------------------------------------------------------------ stage pieces
-  // u, omega -> physical u (to uphys) and the z/y-forward of u x omega in w6[0..3).
-  int nonlinear(const C<T>* usp, T* uphys) {
-    hipLaunchKernelGGL((k_x_uw<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, usp, w6, CF, (const C<T>*)tw, inv());
+  // Flow chain on stream s: u, omega -> physical u (uphys), z/y-forward of
+  // u x omega in w6[0..3).
+  int nonlinear(hipStream_t s, const C<T>* usp, T* uphys) {
+    LD_TRY(prof_mark(LD_K_XUW, s));
+    hipLaunchKernelGGL((k_x_uw<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, s, usp, w6, CF, (const C<T>*)tw, inv());
     LD_LAUNCHED();
-    LD_TRY((pass<1, +1>(w6, w6, 6, inv())));
+    LD_TRY(prof_mark(LD_K_XUW, s));
+    LD_TRY((pass<1, +1>(s, w6, w6, 6, inv())));
     OpZCross<T, N, P6> oc{w6, CF, uphys, inv()};
-    LD_TRY((zrun<P6, 6, 3>(oc)));
-    LD_TRY((pass<1, -1>(w6, w6, 3, (T)1)));
+    LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS)));
+    LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1)));
     return LD_OK;
   }
-  // v_hat (x-inverse, in w6[0..3)) -> physical v and z/y... up to the x-inverse
-  // of v (leaves nv forward-z in w6[0..3)).
-  int velocity_nv(T* vphys, const T* ns, const double* clip) {
-    LD_TRY((pass<1, +1>(w6, w6, 3, inv())));
-    OpZVel<T, N, P3> ov{w6, CF, vphys, ns, clip, inv(), (T)(p.tps * p.g)};
-    LD_TRY((zrun<P3, 3, 3>(ov)));
-    return LD_OK;
+  // v_hat (x-inverse in w6[0..3)): y-inverse; then z-pass to physical v, and n v
+  // z-forward into wn[0..3).
+  int velocity_y(hipStream_t s) { return pass<1, +1>(s, w6, w6, 3, inv()); }
+  int velocity_z(hipStream_t s, T* vphys, const T* ns, const double* clip) {
+    OpZVel<T, N, P3> ov{w6, wn, CF, vphys, ns, clip, inv(), (T)(p.tps * p.g)};
+    return zrun<P3, 3, 3>(s, ov, LD_K_ZVEL);
   }
-  // fc -> z forward into w6[3]; y forward of w6[0..4); kn_hat x-inverse in w6[0]; y-inverse.
-  int density_rhs(const T* fcin) {
-    OpZFwd<T, N, P1, 1> of{fcin, w6 + 3 * CF, RF, CF};
-    LD_TRY((zrun<P1, 0, 1>(of)));
-    LD_TRY((pass<1, -1>(w6, w6, 4, (T)1)));
-    hipLaunchKernelGGL((k_x_kn<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, w6, CF, (const C<T>*)tw, inv());
+  // Density chain on stream s: fc -> z-forward into wn[3]; y-forward of
+  // wn[0..4); kn_hat x-inverse in wn[0]; y-inverse.
+  int density_rhs(hipStream_t s, const T* fcin) {
+    OpZFwd<T, N, P1, 1> of{fcin, wn + 3 * CF, RF, CF};
+    LD_TRY((zrun<P1, 0, 1>(s, of)));
+    LD_TRY((pass<1, -1>(s, wn, wn, 4, (T)1)));
+    LD_TRY(prof_mark(LD_K_XKN, s));
+    hipLaunchKernelGGL((k_x_kn<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, s, wn, CF, (const C<T>*)tw, inv());
     LD_LAUNCHED();
-    LD_TRY((pass<1, +1>(w6, w6, 1, inv())));
+    LD_TRY(prof_mark(LD_K_XKN, s));
+    LD_TRY((pass<1, +1>(s, wn, wn, 1, inv())));
     return LD_OK;
   }
 
   // One RK4 stage s (0..3) of the fused step on the context state.
+  //   stream st  (flow):    forcing factors, u/omega, nonlinear term, RK update
+  //                         of u_hat, v_hat, v (waits for the previous
+  //                         stage's density chain before writing v / nv);
+  //   stream st2 (density): droplets (gather, RHS, deposit, RK update),
+  //                         fc and n v transforms, kn, RK update of n.
+  // The next stage's flow chain depends only on u_hat, so it overlaps this
+  // stage's density chain.  u (physical) is double-buffered for that.
   int stage(int s, double h) {
     static const double a_coef[5] = {0.0, 0.5, 0.5, 1.0, 0.0};
-    static const double w_coef[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     const C<T>* usp = (s == 0) ? uk : us;
     const T* ns = (s == 0) ? n_state : n_stage;
+    T* uphys = ubuf[s & 1];
     const double wh = (s == 0 || s == 3) ? h / 6.0 : h / 3.0;   // forced-dns-sm.py:342-358
     const double anext = a_coef[s + 1] * h;
-    (void)w_coef;
-    LD_TRY(shell(usp, p.isforcing));
-    LD_TRY(nonlinear(usp, u));
+    LD_TRY(shell(st, usp, p.isforcing));
+    LD_TRY(nonlinear(st, usp, uphys));
     RhsArgs<T> ra;
     ra.nl = w6; ra.us = usp; ra.ubase = uk; ra.acc = acc; ra.usnext = (s < 3) ? us : nullptr; ra.ku = nullptr;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
     ra.wacc = (T)wh; ra.anext = (T)anext; ra.scale = inv(); ra.acc_init = (s == 0);
+    LD_TRY(prof_mark(LD_K_XRHS, st));
     hipLaunchKernelGGL((k_x_rhs<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, ra, (const C<T>*)tw);
     LD_LAUNCHED();
-    LD_TRY(velocity_nv(v, ns, clip_stage));
-    LD_TRY(fill<T>(fc, RF, (T)0));
+    LD_TRY(prof_mark(LD_K_XRHS, st));
+    LD_TRY(velocity_y(st));
+    // v, nv need: the previous density chain done (n_s, clip, wn and v free)
+    LD_HIP(hipStreamWaitEvent(st, evB, 0));
+    LD_TRY(velocity_z(st, v, ns, clip_stage));
+    LD_TRY(join(st2, st, evA));
+    LD_TRY(fill<T>(st2, fc, RF, (T)0));
     if (np > 0) {
       PStage<T, PT> ps;
       ps.np = (int)np; ps.cap = (int)cap; ps.stage = s;
       ps.x0 = x0; ps.v0 = v0; ps.sx = sx; ps.sv = sv; ps.kx = kxp; ps.kv = kvp;
-      ps.u = u; ps.v = v; ps.n = ns; ps.clip = clip_stage; ps.fc = fc;
+      ps.u = uphys; ps.v = v; ps.n = ns; ps.clip = clip_stage; ps.fc = fc;
       ps.ah = a_coef[s] * h; ps.wh = wh; ps.P = dp;
       // 4 lanes per droplet; block count a multiple of 8 for the XCD mapping
       const unsigned nb = (unsigned)(((4 * np + 255) / 256 + 7) / 8 * 8);
-      hipLaunchKernelGGL((k_particles_stage<T, PT, N>), dim3(nb), dim3(256), 0, st, ps);
+      LD_TRY(prof_mark(LD_K_DROPLETS, st2));
+      hipLaunchKernelGGL((k_particles_stage<T, PT, N>), dim3(nb), dim3(256), 0, st2, ps);
       LD_LAUNCHED();
+      LD_TRY(prof_mark(LD_K_DROPLETS, st2));
     }
-    LD_TRY(density_rhs(fc));
+    LD_TRY(density_rhs(st2, fc));
     OpZKn<T, N, P1> ok;
-    ok.w = w6; ok.fs = CF; ok.scale = inv(); ok.kn_out = nullptr;
+    ok.w = wn; ok.fs = CF; ok.scale = inv(); ok.kn_out = nullptr;
     ok.n0 = n_state; ok.nacc = nacc; ok.nnext = (s < 3) ? n_stage : nullptr;
     ok.wacc = (T)wh; ok.anext = (T)anext; ok.acc_init = (s == 0);
-    ok.sum_target = (s < 3) ? 0 : 1; ok.partials = partials;
-    LD_TRY((zrun<P1, 1, 0>(ok)));
-    hipLaunchKernelGGL(k_clip_final, dim3(1), dim3(256), 0, st, (const double*)partials, NBLK_Z1,
+    ok.sum_target = (s < 3) ? 0 : 1; ok.partials = partials2;
+    LD_TRY((zrun<P1, 1, 0>(st2, ok, LD_K_ZKN)));
+    hipLaunchKernelGGL(k_clip_final, dim3(1), dim3(256), 0, st2, (const double*)partials2, NBLK_Z1,
                        1.0 / ((double)N * N * N), (s < 3) ? clip_stage : clip_end);
     LD_LAUNCHED();
+    LD_HIP(hipEventRecord(evB, st2));
+    return LD_OK;
+  }
+
+  // Stage-1 clip_zero(n) sums (forced-dns-sm.py:341) on the density stream.
+  int begin_rk4() {
+    LD_TRY(clip_sums(st2, n_state, clip_stage));
+    LD_HIP(hipEventRecord(evB, st2));
     return LD_OK;
   }
 
   // RK4 over the context state: acc <- uknew, nacc <- nnew, droplet base <- sump.
   int rk4_stages(double h) {
-    LD_TRY(clip_sums(n_state, clip_stage));   // stage-1 clip_zero(n) (forced-dns-sm.py:341)
+    LD_TRY(begin_rk4());
     for (int s = 0; s < 4; ++s) LD_TRY(stage(s, h));
     return LD_OK;
   }
 
-  // Step epilogue (forced-dns-sm.py:583-620).
+  // Step epilogue (forced-dns-sm.py:583-620): n = clip_zero(nnew) on the
+  // density stream; reality round trip, projection and blow-up test on the
+  // flow stream.
   int epilogue() {
-    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st, nacc, n_state, RF, (const double*)clip_end, 1);
+    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st2, nacc, n_state, RF, (const double*)clip_end, 1);
     LD_LAUNCHED();
-    LD_TRY((pass<0, +1>(acc, w6, 3, inv())));
-    LD_TRY((pass<1, +1>(w6, w6, 3, inv())));
+    LD_TRY((pass<0, +1>(st, acc, w6, 3, inv())));
+    LD_TRY((pass<1, +1>(st, w6, w6, 3, inv())));
     OpZRound<T, N, P3> orr{w6, CF, inv()};
-    LD_TRY((zrun<P3, 3, 3>(orr)));
-    LD_TRY((pass<1, -1>(w6, w6, 3, (T)1)));
+    LD_TRY((zrun<P3, 3, 3>(st, orr)));
+    LD_TRY((pass<1, -1>(st, w6, w6, 3, (T)1)));
     hipLaunchKernelGGL((k_x_proj<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, (const C<T>*)w6, uk, CF,
                        (const C<T>*)tw, maxpart);
     LD_LAUNCHED();
@@ -355,31 +445,35 @@ struct Engine final : EngineBase {
     return LD_OK;
   }
   int state_set(const void* ukin, const void* nin, const double* coord, int64_t n_) override {
+    LD_TRY(join(st, st2, evB));
     LD_HIP(hipMemcpyAsync(uk, ukin, 3 * CF * sizeof(C<T>), hipMemcpyDeviceToDevice, st));
     LD_HIP(hipMemcpyAsync(n_state, nin, RF * sizeof(T), hipMemcpyDeviceToDevice, st));
     return set_rows(coord, n_);
   }
   int state_get(void* ukout, void* nout, double* coord) override {
+    LD_TRY(join(st, st2, evB));
     if (ukout) LD_HIP(hipMemcpyAsync(ukout, uk, 3 * CF * sizeof(C<T>), hipMemcpyDeviceToDevice, st));
     if (nout) LD_HIP(hipMemcpyAsync(nout, n_state, RF * sizeof(T), hipMemcpyDeviceToDevice, st));
     if (coord) LD_TRY(get_rows(coord));
     return LD_OK;
   }
-  // Counting sort of the droplet slots by spatial bin of the base position.
-  int sort_droplets() {
+
+  // Counting sort of the droplet slots by spatial bin of the base position
+  // (on the density stream, which owns the droplet state during a step).
+  int sort_droplets(hipStream_t s) {
     if (np < 2 || !sort_enabled) return LD_OK;
     const unsigned pb = (unsigned)((np + 255) / 256);
-    LD_TRY(fill<int>(hist, NB, 0));
-    hipLaunchKernelGGL((k_bin_count<N>), dim3(pb), dim3(256), 0, st, (const double*)x0, (int)np, (int)cap, p.L,
+    LD_TRY(fill<int>(s, hist, NB, 0));
+    hipLaunchKernelGGL((k_bin_count<N>), dim3(pb), dim3(256), 0, s, (const double*)x0, (int)np, (int)cap, p.L,
                        dp.dx, pkey, prank, hist);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_local, dim3(NBS), dim3(1024), 0, st, hist, NB, bsum);
+    hipLaunchKernelGGL(k_scan_local, dim3(NBS), dim3(1024), 0, s, hist, NB, bsum);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, st, bsum, NBS);
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, s, bsum, NBS);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_add, dim3((NB + 255) / 256), dim3(256), 0, st, hist, NB, (const int*)bsum);
+    hipLaunchKernelGGL(k_scan_add, dim3((NB + 255) / 256), dim3(256), 0, s, hist, NB, (const int*)bsum);
     LD_LAUNCHED();
-    hipLaunchKernelGGL((k_bin_scatter<PT>), dim3(pb), dim3(256), 0, st, (int)np, (int)cap, (const int*)pkey,
+    hipLaunchKernelGGL((k_bin_scatter<PT>), dim3(pb), dim3(256), 0, s, (int)np, (int)cap, (const int*)pkey,
                        (const int*)prank, (const int*)hist, (const double*)x0, (const PT*)v0, (const int64_t*)rid,
                        x0b, v0b, rid2);
     LD_LAUNCHED();
@@ -390,11 +484,13 @@ struct Engine final : EngineBase {
   }
 
   int advance(double h, int nsteps) override {
+    LD_TRY(join(st2, st, evA));      // state written on the caller's stream
     for (int i = 0; i < nsteps; ++i) {
-      LD_TRY(sort_droplets());
+      LD_TRY(sort_droplets(st2));
       LD_TRY(rk4_stages(h));
       LD_TRY(epilogue());
     }
+    LD_TRY(join(st, st2, evB));      // caller's stream sees the whole step
     int32_t blown = 0;
     LD_HIP(hipMemcpyAsync(&blown, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     LD_HIP(hipStreamSynchronize(st));
@@ -402,6 +498,7 @@ struct Engine final : EngineBase {
     return LD_OK;
   }
   int status(double* last_max, int32_t* blown) override {
+    LD_TRY(join(st, st2, evB));
     if (last_max) LD_HIP(hipMemcpyAsync(last_max, lastmax, sizeof(double), hipMemcpyDeviceToHost, st));
     if (blown) LD_HIP(hipMemcpyAsync(blown, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     LD_HIP(hipStreamSynchronize(st));
@@ -411,7 +508,9 @@ struct Engine final : EngineBase {
   int rk4(double h, const void* ukin, void* nio, const double* coord, int64_t n_, void* uknew, void* nnew,
           double* coord_new) override {
     LD_TRY(state_set(ukin, nio, coord, n_));
+    LD_TRY(join(st2, st, evA));
     LD_TRY(rk4_stages(h));
+    LD_TRY(join(st, st2, evB));
     LD_HIP(hipMemcpyAsync(uknew, acc, 3 * CF * sizeof(C<T>), hipMemcpyDeviceToDevice, st));
     LD_HIP(hipMemcpyAsync(nnew, nacc, RF * sizeof(T), hipMemcpyDeviceToDevice, st));
     if (coord_new) LD_TRY(get_rows(coord_new));
@@ -420,15 +519,17 @@ struct Engine final : EngineBase {
     return LD_OK;
   }
 
+  // One RHS evaluation with the reference's full_RHS semantics (single stream).
   int full_rhs(const void* ukin, const void* nin, const double* coord, int64_t n_, void* ku, void* kn, double* kp,
                void* fcout, double visc, double forc, void* u_out, void* v_out) override {
     if (n_ > cap) { set_error("%lld droplets exceed the context capacity %lld", (long long)n_, (long long)cap); return LD_ERR_CAPACITY; }
+    LD_TRY(join(st, st2, evB));
     const C<T>* ukp = (const C<T>*)ukin;
-    T* uphys = u_out ? (T*)u_out : u;
+    T* uphys = u_out ? (T*)u_out : ubuf[0];
     T* vphys = v_out ? (T*)v_out : v;
     T* fcp = (T*)fcout;
-    LD_TRY(shell(ukp, p.isforcing * forc));
-    LD_TRY(nonlinear(ukp, uphys));
+    LD_TRY(shell(st, ukp, p.isforcing * forc));
+    LD_TRY(nonlinear(st, ukp, uphys));
     RhsArgs<T> ra;
     ra.nl = w6; ra.us = ukp; ra.ubase = ukp; ra.acc = nullptr; ra.usnext = nullptr; ra.ku = (C<T>*)ku;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit * visc); ra.tps = (T)p.tps;
@@ -436,33 +537,36 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL((k_x_rhs<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, ra, (const C<T>*)tw);
     LD_LAUNCHED();
     // the physical-space v the droplets see uses visc as well (forced-dns-sm.py:319-321)
-    LD_TRY(velocity_nv(vphys, (const T*)nin, clip_none));
-    LD_TRY(fill<T>(fcp, RF, (T)0));
+    LD_TRY(velocity_y(st));
+    LD_TRY(velocity_z(st, vphys, (const T*)nin, clip_none));
+    LD_TRY(fill<T>(st, fcp, RF, (T)0));
     if (n_ > 0) {
       hipLaunchKernelGGL((k_prhs<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, coord, (int)n_,
                          (const T*)uphys, (const T*)vphys, (const T*)nin, (const double*)clip_none, fcp, kp, rows_a,
                          wrow, dp);
       LD_LAUNCHED();
     }
-    LD_TRY(density_rhs(fcp));
+    LD_TRY(density_rhs(st, fcp));
     OpZKn<T, N, P1> ok;
-    ok.w = w6; ok.fs = CF; ok.scale = inv(); ok.kn_out = (T*)kn;
+    ok.w = wn; ok.fs = CF; ok.scale = inv(); ok.kn_out = (T*)kn;
     ok.n0 = nullptr; ok.nacc = nullptr; ok.nnext = nullptr; ok.wacc = (T)0; ok.anext = (T)0; ok.acc_init = 0;
-    ok.sum_target = -1; ok.partials = partials;
-    LD_TRY((zrun<P1, 1, 0>(ok)));
+    ok.sum_target = -1; ok.partials = partials2;
+    LD_TRY((zrun<P1, 1, 0>(st, ok)));
     return LD_OK;
   }
 
   // ------------------------------------------------------------ droplet API
   int interp(const double* coord, int64_t n_, const void* f, int nc, double* out) override {
     if (n_ <= 0) return LD_OK;
+    LD_TRY(join(st, st2, evB));
     hipLaunchKernelGGL((k_interp<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, coord, (int)n_,
                        (const T*)f, nc, out, dp);
     LD_LAUNCHED();
     return LD_OK;
   }
   int exterp(const double* coord, int64_t n_, const double* w, void* c) override {
-    LD_TRY(fill<T>((T*)c, RF, (T)0));
+    LD_TRY(join(st, st2, evB));
+    LD_TRY(fill<T>(st, (T*)c, RF, (T)0));
     if (n_ <= 0) return LD_OK;
     hipLaunchKernelGGL((k_exterp<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, coord, (int)n_, w,
                        (T*)c, dp);
@@ -471,7 +575,8 @@ struct Engine final : EngineBase {
   }
   int prhs(const double* coord, int64_t n_, const void* uf, const void* usf, const void* nf, void* fcout,
            double* rhs, double* interpm, double* exterpm) override {
-    LD_TRY(fill<T>((T*)fcout, RF, (T)0));
+    LD_TRY(join(st, st2, evB));
+    LD_TRY(fill<T>(st, (T*)fcout, RF, (T)0));
     if (n_ <= 0) return LD_OK;
     hipLaunchKernelGGL((k_prhs<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, coord, (int)n_,
                        (const T*)uf, (const T*)usf, (const T*)nf, (const double*)clip_none, (T*)fcout, rhs, interpm,

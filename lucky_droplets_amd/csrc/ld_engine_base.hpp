@@ -26,6 +26,8 @@ struct EngineBase {
   virtual int status(double* last_max, int32_t* blown) = 0;
   virtual int interp(const double* coord, int64_t np, const void* f, int nc, double* out) = 0;
   virtual int exterp(const double* coord, int64_t np, const double* w, void* c) = 0;
+  virtual int profile(int kernel, int enable) = 0;
+  virtual int profile_read(double* total_ms, int64_t* launches) = 0;
   virtual int prhs(const double* coord, int64_t np, const void* u, const void* us, const void* n, void* fc,
                    double* rhs, double* interp, double* exterp) = 0;
 };

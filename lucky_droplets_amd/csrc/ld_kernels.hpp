@@ -340,7 +340,7 @@ __device__ __forceinline__ T nStage(T x, const double* __restrict__ clip) {
 // pass), v stored, then n*v transformed along z (:326-329, first pass).
 template <typename T, int N, int P>
 struct OpZVel {
-  C<T>* w; size_t fs; T* v; const T* n; const double* clip; T scale; T tpsg;
+  const C<T>* w; C<T>* wout; size_t fs; T* v; const T* n; const double* clip; T scale; T tpsg;
   __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
@@ -356,7 +356,7 @@ struct OpZVel {
       t.re(2, p, j) = nn * vz;
     }
   }
-  __device__ void storeSpec(int f, int line, int k, C<T> v_) const { w[f * fs + (size_t)line * (N / 2 + 1) + k] = v_; }
+  __device__ void storeSpec(int f, int line, int k, C<T> v_) const { wout[f * fs + (size_t)line * (N / 2 + 1) + k] = v_; }
 };
 
 // Density RHS kn to physical space (last pass of forced-dns-sm.py:330) fused

@@ -226,6 +226,17 @@ class ForcedDNS:
         self._stream()
         _lib.check(self.lib.ld_advance(self.ctx, float(h), int(nsteps)))
 
+    def profile(self, kernel: str, enable: bool = True):
+        """Time every launch of one kernel class with HIP events (see ld_profile)."""
+        _lib.check(self.lib.ld_profile(self.ctx, _lib.KERNELS[kernel], 1 if enable else 0))
+
+    def profile_read(self):
+        """(total_ms, launches) since profile() was enabled."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        _lib.check(self.lib.ld_profile_read(self.ctx, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def status(self):
         m = ctypes.c_double()
         b = ctypes.c_int32()

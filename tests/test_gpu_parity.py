@@ -153,3 +153,36 @@ def test_steps(gpu, prec):
     assert min_image(a, b, cfg.L) < tol * cfg.L
     assert relerr(a[:, 3:], b[:, 3:]) < tol
     d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_steps_dense_droplets(gpu, prec):
+    """Many droplets per cell: exercises the spatial sort, shared deposit nodes
+    and the overlapped density stream; rows come back in input order."""
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    from lucky_droplets_amd.particles import MPI_particles
+    nparts = 3000
+    cfg = _cfg(16, prec, nparts, nu=0.05)
+    uk, n, coord, pid = inputs(cfg, seed=9)
+    fl = make_flow_oracle(cfg)
+    dr = make_drops_oracle(cfg, coord, pid)
+    h = 0.01
+    uk_r, n_r = uk.copy(), n.copy()
+    for _ in range(4):
+        uk_r, n_r = fl.step(0.0, h, dr, uk_r, n_r)
+    d = ForcedDNS(cfg)
+    stb = MPI_particles(None, cfg.L, cfg.N, nparts, cfg.sts, 0.05, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0,
+                        dns=d, coord=coord, prtclid=pid)
+    ukg = _t(uk, gpu, d.cdtype)
+    ng = _t(n, gpu, d.rdtype)
+    d.step(0.0, h, stb, ukg, ng, nsteps=2)
+    d.step(0.0, h, stb, ukg, ng, nsteps=2)
+    tol = TOL[prec]
+    assert relerr(ukg.cpu().numpy(), uk_r) < tol
+    assert relerr(ng.cpu().numpy(), n_r) < tol
+    a = stb.coord.cpu().numpy()               # input row order is kept
+    b = by_id(dr.coord, dr.prtclid)
+    assert min_image(a, b, cfg.L) < tol * cfg.L
+    assert relerr(a[:, 3:], b[:, 3:]) < tol
+    d.close()
