@@ -82,6 +82,26 @@ def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None):
     return ms, kern, lastmax, blown
 
 
+def time_plain(cfg, uk, n, coord, dt, steps):
+    """ms per step of the unfused reference-dataflow path (ld_advance_plain)."""
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    dev = torch.device("cuda:0")
+    d = ForcedDNS(cfg)
+    d.set_state(torch.from_numpy(uk).to(dev).to(d.cdtype), torch.from_numpy(n).to(dev).to(d.rdtype),
+                torch.from_numpy(coord).to(dev))
+    d.advance_plain(dt, 1)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    d.advance_plain(dt, steps)
+    e1.record()
+    torch.cuda.synchronize()
+    d.close()
+    return e0.elapsed_time(e1) / steps
+
+
 def droplet_launch_bytes(N, s, npart, sp):
     """Algorithmic bytes of one droplet-stage launch (one quarter of the
     per-step droplet terms): min(7R, 448 s Np) + R + 5 Np S_p."""
@@ -187,7 +207,14 @@ def main():
         log(f"[bench] FP64 timing: steps {nsteps64}")
         ms64, _, _, _ = time_gpu(c64, uk, n, coord, dt, nsteps64, 1)
         log(f"[bench] FP64 {ms64:.3f} ms/step")
-        out["fp64"] = {"ms_per_step": ms64, "steps_per_s": 1000.0 / ms64, "fp32_speedup": ms64 / ms32}
+        out["fp64_fused"] = {"ms_per_step": ms64, "steps_per_s": 1000.0 / ms64, "fp32_speedup": ms64 / ms32}
+        nplain = max(2, args.steps // 5)
+        log(f"[bench] FP64 plain timing: steps {nplain}")
+        msp = time_plain(c64, uk, n, coord, dt, nplain)
+        log(f"[bench] FP64 plain {msp:.3f} ms/step")
+        out["fp64_plain"] = {"ms_per_step": msp, "steps_per_s": 1000.0 / msp, "fp32_speedup": msp / ms32,
+                             "form": "reference dataflow on the GPU: 74 separate 3-D transforms per step and one "
+                                     "kernel per array statement (ld_advance_plain)"}
     out["blowup"] = bool(blown)
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, bounded sample)")

@@ -110,6 +110,11 @@ int ld_step(ld_ctx* ctx, double h, void* uk, void* n, double* coord, int64_t np,
 int ld_state_set(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np);
 int ld_state_get(ld_ctx* ctx, void* uk, void* n, double* coord);
 int ld_advance(ld_ctx* ctx, double h, int32_t nsteps);
+/* The same steps with the reference's dataflow, unfused ("plain" form): every
+ * 3-D transform separate (74 per step) and one kernel per array statement of
+ * RK4/full_RHS and the loop body (forced-dns-sm.py:287-362, 582-620); droplet
+ * rows are processed in input order.  The baseline of the fused path. */
+int ld_advance_plain(ld_ctx* ctx, double h, int32_t nsteps);
 /* Last step's max Re(uk) (host double) and sticky blow-up flag. */
 int ld_status(ld_ctx* ctx, double* last_max, int32_t* blown_up);
 
@@ -143,6 +148,23 @@ int ld_exterp_cosine_scalar(ld_ctx* ctx, const double* coord, int64_t np, const 
  * Also returns interpmat (np, 7) and exterpmat (np).  n is used as given. */
 int ld_prhs(ld_ctx* ctx, const double* coord, int64_t np, const void* u, const void* us, const void* n,
             void* fc, double* rhs, double* interpmat, double* exterpmat);
+
+/* ---------------------------------------------------------- collisions */
+/* Droplet-droplet collision pairs with a cell list.  No counterpart in the
+ * reference (its droplets grow only by collecting the continuum density,
+ * particles.py:828).  Pairs (i, j), i < j, of rows whose minimum-image FP64
+ * distance satisfies d^2 < (r_i + r_j)^2 (d2 = (dx*dx + dy*dy) + dz*dz, no
+ * fused multiply-add), sorted by (i, j), written to pairs (max_pairs, 2) int64.
+ * rmax >= every radius.  *npairs (HOST) receives the count; LD_ERR_CAPACITY
+ * if it exceeds max_pairs. */
+int ld_collide_pairs(ld_ctx* ctx, const double* rows, const double* radius, int64_t np, double rmax,
+                     int64_t* pairs, int64_t max_pairs, int64_t* npairs);
+/* Coalescence events for sorted pairs: visited in order, a pair merges only if
+ * neither droplet merged earlier in this call; survivor i takes the summed
+ * mass, mass-weighted velocity and position x_i + (m_j/m) d; row j is left
+ * with zero mass and velocity.  events (npairs, 2) int64; *nevents (HOST). */
+int ld_coalesce(ld_ctx* ctx, double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
+                int64_t* nevents);
 
 #ifdef __cplusplus
 }

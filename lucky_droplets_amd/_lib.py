@@ -73,12 +73,16 @@ _SIGS = {
     "ld_state_set": [_VP, _VP, _VP, _VP, ctypes.c_int64],
     "ld_state_get": [_VP, _VP, _VP, _VP],
     "ld_advance": [_VP, ctypes.c_double, ctypes.c_int32],
+    "ld_advance_plain": [_VP, ctypes.c_double, ctypes.c_int32],
     "ld_status": [_VP, _c_double_p, ctypes.POINTER(ctypes.c_int32)],
     "ld_profile": [_VP, ctypes.c_int32, ctypes.c_int32],
     "ld_profile_read": [_VP, _c_double_p, ctypes.POINTER(ctypes.c_int64)],
     "ld_interp_cosine": [_VP, _VP, ctypes.c_int64, _VP, ctypes.c_int32, _VP],
     "ld_exterp_cosine_scalar": [_VP, _VP, ctypes.c_int64, _VP, _VP],
     "ld_prhs": [_VP, _VP, ctypes.c_int64, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
+    "ld_collide_pairs": [_VP, _VP, _VP, ctypes.c_int64, ctypes.c_double, _VP, ctypes.c_int64,
+                         ctypes.POINTER(ctypes.c_int64)],
+    "ld_coalesce": [_VP, _VP, ctypes.c_int64, _VP, ctypes.c_int64, _VP, ctypes.POINTER(ctypes.c_int64)],
 }
 EXPORTS = tuple(_SIGS) + ("ld_version",)
 

@@ -161,6 +161,12 @@ int ld_advance(ld_ctx* ctx, double h, int32_t nsteps) {
   return ctx->eng->advance(h, nsteps);
 }
 
+int ld_advance_plain(ld_ctx* ctx, double h, int32_t nsteps) {
+  LD_CHECK_CTX(ctx);
+  if (nsteps < 0) { set_error("nsteps must be >= 0"); return LD_ERR_ARG; }
+  return ctx->eng->advance_plain(h, nsteps);
+}
+
 int ld_step(ld_ctx* ctx, double h, void* uk, void* n, double* coord, int64_t np, int32_t nsteps) {
   LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n);
   if (np < 0 || nsteps < 0) { set_error("np and nsteps must be >= 0"); return LD_ERR_ARG; }
@@ -188,6 +194,22 @@ int ld_profile(ld_ctx* ctx, int32_t kernel, int32_t enable) {
 int ld_profile_read(ld_ctx* ctx, double* total_ms, int64_t* launches) {
   LD_CHECK_CTX(ctx); LD_NONNULL(total_ms); LD_NONNULL(launches);
   return ctx->eng->profile_read(total_ms, launches);
+}
+
+int ld_collide_pairs(ld_ctx* ctx, const double* rows, const double* radius, int64_t np, double rmax,
+                     int64_t* pairs, int64_t max_pairs, int64_t* npairs) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(npairs);
+  if (np < 0 || max_pairs < 0) { set_error("np and max_pairs must be >= 0"); return LD_ERR_ARG; }
+  if (np > 1) { LD_NONNULL(rows); LD_NONNULL(radius); LD_NONNULL(pairs); }
+  return ctx->eng->collide(rows, radius, np, rmax, pairs, max_pairs, npairs);
+}
+
+int ld_coalesce(ld_ctx* ctx, double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
+                int64_t* nevents) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(nevents);
+  if (np < 0 || npairs < 0) { set_error("np and npairs must be >= 0"); return LD_ERR_ARG; }
+  if (npairs > 0) { LD_NONNULL(rows); LD_NONNULL(pairs); LD_NONNULL(events); }
+  return ctx->eng->coalesce(rows, np, pairs, npairs, events, nevents);
 }
 
 int ld_interp_cosine(ld_ctx* ctx, const double* coord, int64_t np, const void* fields, int32_t ncomp, double* out) {

@@ -3,6 +3,8 @@
 #pragma once
 #include "ld_engine_base.hpp"
 #include "ld_particles.hpp"
+#include "ld_plain.hpp"
+#include "ld_collide.hpp"
 #include <cmath>
 #include <utility>
 #include <vector>
@@ -552,6 +554,247 @@ struct Engine final : EngineBase {
     ok.n0 = nullptr; ok.nacc = nullptr; ok.nnext = nullptr; ok.wacc = (T)0; ok.anext = (T)0; ok.acc_init = 0;
     ok.sum_target = -1; ok.partials = partials2;
     LD_TRY((zrun<P1, 1, 0>(st, ok)));
+    return LD_OK;
+  }
+
+  // ------------------------------------------------------------ plain form
+  // The reference's dataflow, unfused: every 3-D transform separate and one
+  // kernel per array statement of forced-dns-sm.py:287-362 and 582-620, on
+  // the caller's stream, droplets as (Np, 7) rows without reordering.
+  T *pr3 = nullptr, *knp = nullptr;
+  C<T> *kbuf = nullptr, *scr3 = nullptr;
+  double *rows_c = nullptr, *rows_d = nullptr, *rows_e = nullptr, *rows_f = nullptr, *maxpart2 = nullptr;
+  static constexpr int NBLK_SPEC = (int)((CF + 255) / 256);
+
+  int plain_alloc() {
+    if (pr3) return LD_OK;
+    LD_TRY(alloc(pr3, 3 * RF)); LD_TRY(alloc(knp, RF)); LD_TRY(alloc(kbuf, 3 * CF)); LD_TRY(alloc(scr3, 3 * CF));
+    LD_TRY(alloc(maxpart2, NBLK_SPEC));
+    if (cap > 0) {
+      LD_TRY(alloc(rows_c, 7 * cap)); LD_TRY(alloc(rows_d, 7 * cap));
+      LD_TRY(alloc(rows_e, 7 * cap)); LD_TRY(alloc(rows_f, 7 * cap));
+    }
+    return LD_OK;
+  }
+  static dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+  // rfft_mpi: z, y, x passes (forced-dns-sm.py:201-206)
+  int p_rfft(const T* in, C<T>* out, int nf) {
+    for (int f = 0; f < nf; ++f) {
+      OpZFwd<T, N, P1, 1> op{in + f * RF, out + f * CF, RF, CF};
+      LD_TRY((zrun<P1, 0, 1>(st, op)));
+    }
+    LD_TRY((pass<1, -1>(st, out, out, nf, (T)1)));
+    LD_TRY((pass<0, -1>(st, out, out, nf, (T)1)));
+    return LD_OK;
+  }
+  // irfft_mpi: x, y, z passes (forced-dns-sm.py:208-213), nf <= 3
+  int p_irfft(const C<T>* in, T* out, int nf) {
+    LD_TRY((pass<0, +1>(st, in, scr3, nf, inv())));
+    LD_TRY((pass<1, +1>(st, scr3, scr3, nf, inv())));
+    for (int f = 0; f < nf; ++f) {
+      OpZInv<T, N, P1, 1> op{scr3 + f * CF, out + f * RF, CF, RF, inv()};
+      LD_TRY((zrun<P1, 1, 0>(st, op)));
+    }
+    return LD_OK;
+  }
+  template <class X, class S> int p_axpy(const X* x, const X* z, X* y, size_t n, S a) {
+    hipLaunchKernelGGL((k_pl_axpy<X, S>), g1(n), dim3(256), 0, st, x, z, y, n, a);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  int p_clip_zero(T* x, T* out) {   // clip_zero (:278-285): sums, clip x in place, out = x*old/new
+    LD_TRY(clip_sums(st, x, clip_end));
+    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st, x, out, RF, (const double*)clip_end, 1);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+
+  // full_RHS (:287-332) on stage inputs; ku -> kbuf, kn -> knp, kp -> rows_e.
+  int plain_rhs(const C<T>* ukp, const T* nsp, const double* coord_s) {
+    C<T>* fk = wn;
+    T* u_ = ubuf[0];
+    T* om = ubuf[1];
+    LD_TRY(shell(st, ukp, p.isforcing));                                                         // :289
+    hipLaunchKernelGGL((k_forcing_apply<T, N>), g1(CF), dim3(256), 0, st, ukp, fk, CF, (const double*)fac);
+    LD_LAUNCHED();
+    LD_TRY(p_irfft(ukp, u_, 3));                                                                  // :291-293
+    hipLaunchKernelGGL((k_pl_curl<T, N>), g1(CF), dim3(256), 0, st, ukp, w6, CF);                  // :295-297
+    LD_LAUNCHED();
+    LD_TRY(p_irfft(w6, om, 3));
+    hipLaunchKernelGGL((k_pl_cross<T>), g1(RF), dim3(256), 0, st, (const T*)u_, (const T*)om, pr3, RF);  // :299-301
+    LD_LAUNCHED();
+    C<T>* rk = w6 + 3 * CF;
+    LD_TRY(p_rfft(pr3, rk, 3));                                                                    // :304-306
+    hipLaunchKernelGGL((k_pl_add_mask<T, N>), g1(CF), dim3(256), 0, st, rk, (const C<T>*)fk, CF);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL((k_pl_pressure<T, N>), g1(CF), dim3(256), 0, st, (const C<T>*)rk, ukp, kbuf, w6, CF,
+                       (T)(p.nu * p.isexplicit), (T)p.tps);                                        // :309-321
+    LD_LAUNCHED();
+    LD_TRY(p_irfft(w6, v, 3));
+    hipLaunchKernelGGL((k_pl_shift<T>), g1(RF), dim3(256), 0, st, v + 2 * RF, RF, (T)(p.tps * p.g));
+    LD_LAUNCHED();
+    LD_TRY(fill<T>(st, fc, RF, (T)0));
+    if (np > 0) {                                                                                   // :324
+      hipLaunchKernelGGL((k_prhs<T, N>), g1(np), dim3(256), 0, st, coord_s, (int)np, (const T*)u_, (const T*)v, nsp,
+                         (const double*)clip_none, fc, rows_e, rows_a, wrow, dp);
+      LD_LAUNCHED();
+    }
+    LD_TRY(p_rfft(fc, wn + 3 * CF, 1));                                                             // :325
+    hipLaunchKernelGGL((k_pl_mask<T, N>), g1(CF), dim3(256), 0, st, wn + 3 * CF, CF, 1);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL((k_pl_nv<T>), g1(RF), dim3(256), 0, st, nsp, (const T*)v, pr3, RF);          // :326
+    LD_LAUNCHED();
+    LD_TRY(p_rfft(pr3, w6, 3));                                                                      // :327-329
+    hipLaunchKernelGGL((k_pl_mask<T, N>), g1(CF), dim3(256), 0, st, w6, CF, 3);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL((k_pl_div<T, N>), g1(CF), dim3(256), 0, st, (const C<T>*)w6, (const C<T>*)(wn + 3 * CF), wn,
+                       CF);                                                                          // :330
+    LD_LAUNCHED();
+    LD_TRY(p_irfft(wn, knp, 1));
+    return LD_OK;
+  }
+
+  int advance_plain(double h, int nsteps) override {
+    LD_TRY(plain_alloc());
+    LD_TRY(join(st, st2, evB));
+    const size_t nr = 7 * (size_t)np;
+    if (np > 0) LD_TRY(get_rows(rows_c));        // droplet rows in input order
+    static const double a_coef[4] = {0.0, 0.5, 0.5, 1.0};
+    for (int it = 0; it < nsteps; ++it) {
+      // RK4 (:335-362)
+      if (np > 0) LD_HIP(hipMemcpyAsync(rows_d, rows_c, nr * sizeof(double), hipMemcpyDeviceToDevice, st));  // :337
+      LD_HIP(hipMemcpyAsync(acc, uk, 3 * CF * sizeof(C<T>), hipMemcpyDeviceToDevice, st));                  // :338
+      LD_HIP(hipMemcpyAsync(nacc, n_state, RF * sizeof(T), hipMemcpyDeviceToDevice, st));                   // :339
+      for (int s = 0; s < 4; ++s) {
+        const C<T>* ukp = uk;
+        const double* cs = rows_c;
+        if (s == 0) {
+          LD_TRY(p_clip_zero(n_state, n_stage));                                                           // :341
+        } else {
+          const double a = a_coef[s] * h;
+          LD_TRY(p_axpy(uk, (const C<T>*)kbuf, us, 3 * CF, (T)a));                                         // uk + ku*h/2
+          LD_TRY(p_axpy(n_state, (const T*)knp, pr3, RF, (T)a));                                           // n + kn*h/2
+          LD_TRY(p_clip_zero(pr3, n_stage));
+          if (np > 0) LD_TRY(p_axpy(rows_c, (const double*)rows_e, rows_f, nr, a));                        // coord + h/2 kp
+          ukp = us;
+          cs = rows_f;
+        }
+        LD_TRY(plain_rhs(ukp, n_stage, cs));
+        const double w = (s == 0 || s == 3) ? h / 6.0 : h / 3.0;
+        if (np > 0) LD_TRY(p_axpy(rows_d, (const double*)rows_e, rows_d, nr, w));                          // sump += w kp
+        LD_TRY(p_axpy(acc, (const C<T>*)kbuf, acc, 3 * CF, (T)w));                                         // uknew += w ku
+        LD_TRY(p_axpy(nacc, (const T*)knp, nacc, RF, (T)w));                                               // nnew += w kn
+      }
+      // epilogue (:582-620)
+      if (np > 0) LD_HIP(hipMemcpyAsync(rows_c, rows_d, nr * sizeof(double), hipMemcpyDeviceToDevice, st));  // :582
+      LD_TRY(p_clip_zero(nacc, n_state));                                                                  // :584
+      LD_TRY(p_irfft(acc, ubuf[0], 3));                                                                    // :597-599
+      LD_TRY(p_rfft(ubuf[0], uk, 3));                                                                      // :601-603
+      hipLaunchKernelGGL((k_pl_project<T, N>), dim3(NBLK_SPEC), dim3(256), 0, st, uk, CF, maxpart2);       // :607-610
+      LD_LAUNCHED();
+      hipLaunchKernelGGL(k_max_final_n, dim3(1), dim3(256), 0, st, (const double*)maxpart2, NBLK_SPEC,
+                         100.0 * (double)N * N * N, flag, lastmax);                                         // :618
+      LD_LAUNCHED();
+    }
+    if (np > 0) LD_TRY(set_rows(rows_c, np));
+    int32_t blown = 0;
+    LD_HIP(hipMemcpyAsync(&blown, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    LD_HIP(hipStreamSynchronize(st));
+    if (blown) { set_error("Threshold exceeded: max Re(u_hat) > 100 N^3"); return LD_ERR_BLOWUP; }
+    return LD_OK;
+  }
+
+  // ------------------------------------------------------------ collisions
+  int *ckey = nullptr, *crank = nullptr, *corder = nullptr, *chist = nullptr, *cbsum = nullptr, *cused = nullptr;
+  unsigned long long *ckeys = nullptr, *ccount = nullptr;
+  int *cnev = nullptr;
+  int64_t ccap = 0, ckcap = 0;
+  int chist_cap = 0;
+
+  static int64_t pow2ceil(int64_t x) { int64_t v = 2; while (v < x) v <<= 1; return v; }
+
+  int collide(const double* rows, const double* rad, int64_t n_, double rmax, int64_t* pairs, int64_t max_pairs,
+              int64_t* count) override {
+    *count = 0;
+    if (n_ < 2) return LD_OK;
+    if (n_ > 0x7fffffff) { set_error("too many droplets"); return LD_ERR_ARG; }
+    LD_TRY(join(st, st2, evB));
+    // cells of side >= 2 rmax; fewer than 3 per side -> one cell (all pairs)
+    int nc = (rmax > 0.0) ? (int)floor(p.L / (2.0 * rmax)) : 256;
+    if (nc > 256) nc = 256;
+    if (nc < 3) nc = 1;
+    const double cell = p.L / nc;
+    const int ncells = nc * nc * nc;
+    if (n_ > ccap) {
+      LD_TRY(alloc(ckey, n_)); LD_TRY(alloc(crank, n_)); LD_TRY(alloc(corder, n_)); LD_TRY(alloc(cused, n_));
+      ccap = n_;
+    }
+    if (ncells > chist_cap) {
+      LD_TRY(alloc(chist, ncells)); LD_TRY(alloc(cbsum, (ncells + 4095) / 4096));
+      chist_cap = ncells;
+    }
+    const int64_t kc = pow2ceil(max_pairs > 2 ? max_pairs : 2);
+    if (kc > ckcap) { LD_TRY(alloc(ckeys, kc)); ckcap = kc; }
+    if (!ccount) { LD_TRY(alloc(ccount, 1)); LD_TRY(alloc(cnev, 1)); }
+    const unsigned pb = (unsigned)((n_ + 255) / 256);
+    LD_TRY(fill<int>(st, chist, ncells, 0));
+    hipLaunchKernelGGL(k_coll_count, dim3(pb), dim3(256), 0, st, rows, (int)n_, p.L, cell, nc, ckey, crank, chist);
+    LD_LAUNCHED();
+    const int nbs = (ncells + 4095) / 4096;
+    hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(1024), 0, st, chist, ncells, cbsum);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, st, cbsum, nbs);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_add, dim3((ncells + 255) / 256), dim3(256), 0, st, chist, ncells, (const int*)cbsum);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL(k_coll_scatter, dim3(pb), dim3(256), 0, st, (int)n_, (const int*)ckey, (const int*)crank,
+                       (const int*)chist, corder);
+    LD_LAUNCHED();
+    LD_TRY(fill<unsigned long long>(st, ccount, 1, 0ull));
+    hipLaunchKernelGGL(k_coll_pairs, dim3(pb), dim3(256), 0, st, rows, rad, (int)n_, p.L, cell, nc,
+                       (const int*)chist, (const int*)corder, (const int*)ckey, ckeys,
+                       (unsigned long long)max_pairs, ccount);
+    LD_LAUNCHED();
+    unsigned long long cnt = 0;
+    LD_HIP(hipMemcpyAsync(&cnt, ccount, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    LD_HIP(hipStreamSynchronize(st));
+    *count = (int64_t)cnt;
+    if ((int64_t)cnt > max_pairs) {
+      set_error("%llu collision pairs exceed max_pairs = %lld", cnt, (long long)max_pairs);
+      return LD_ERR_CAPACITY;
+    }
+    if (cnt == 0) return LD_OK;
+    const int n2 = (int)pow2ceil((int64_t)cnt);
+    if ((int64_t)n2 > (int64_t)cnt)
+      LD_TRY(fill<unsigned long long>(st, ckeys + cnt, (size_t)(n2 - (int64_t)cnt), ~0ull));
+    for (int k = 2; k <= n2; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        hipLaunchKernelGGL(k_bitonic, dim3((n2 + 255) / 256), dim3(256), 0, st, ckeys, n2, j, k);
+        LD_LAUNCHED();
+      }
+    hipLaunchKernelGGL(k_pairs_out, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                       (const unsigned long long*)ckeys, (int)cnt, pairs);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+
+  int coalesce(double* rows, int64_t n_, const int64_t* pairs, int64_t npairs, int64_t* events,
+               int64_t* nevents) override {
+    *nevents = 0;
+    if (npairs <= 0) return LD_OK;
+    if (n_ > ccap) {
+      LD_TRY(alloc(ckey, n_)); LD_TRY(alloc(crank, n_)); LD_TRY(alloc(corder, n_)); LD_TRY(alloc(cused, n_));
+      ccap = n_;
+    }
+    if (!ccount) { LD_TRY(alloc(ccount, 1)); LD_TRY(alloc(cnev, 1)); }
+    LD_TRY(join(st, st2, evB));
+    LD_TRY(fill<int>(st, cused, (size_t)n_, 0));
+    hipLaunchKernelGGL(k_coalesce, dim3(1), dim3(64), 0, st, rows, pairs, (int)npairs, p.L, cused, events, cnev);
+    LD_LAUNCHED();
+    int ne = 0;
+    LD_HIP(hipMemcpyAsync(&ne, cnev, sizeof(int), hipMemcpyDeviceToHost, st));
+    LD_HIP(hipStreamSynchronize(st));
+    *nevents = ne;
     return LD_OK;
   }
 

@@ -23,9 +23,14 @@ struct EngineBase {
   virtual int state_set(const void* uk, const void* n, const double* coord, int64_t np) = 0;
   virtual int state_get(void* uk, void* n, double* coord) = 0;
   virtual int advance(double h, int nsteps) = 0;
+  virtual int advance_plain(double h, int nsteps) = 0;
   virtual int status(double* last_max, int32_t* blown) = 0;
   virtual int interp(const double* coord, int64_t np, const void* f, int nc, double* out) = 0;
   virtual int exterp(const double* coord, int64_t np, const double* w, void* c) = 0;
+  virtual int collide(const double* rows, const double* rad, int64_t np, double rmax, int64_t* pairs,
+                      int64_t max_pairs, int64_t* count) = 0;
+  virtual int coalesce(double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
+                       int64_t* nevents) = 0;
   virtual int profile(int kernel, int enable) = 0;
   virtual int profile_read(double* total_ms, int64_t* launches) = 0;
   virtual int prhs(const double* coord, int64_t np, const void* u, const void* us, const void* n, void* fc,

@@ -226,6 +226,11 @@ class ForcedDNS:
         self._stream()
         _lib.check(self.lib.ld_advance(self.ctx, float(h), int(nsteps)))
 
+    def advance_plain(self, h, nsteps: int = 1):
+        """nsteps with the reference's unfused dataflow (ld_advance_plain)."""
+        self._stream()
+        _lib.check(self.lib.ld_advance_plain(self.ctx, float(h), int(nsteps)))
+
     def profile(self, kernel: str, enable: bool = True):
         """Time every launch of one kernel class with HIP events (see ld_profile)."""
         _lib.check(self.lib.ld_profile(self.ctx, _lib.KERNELS[kernel], 1 if enable else 0))

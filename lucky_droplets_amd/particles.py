@@ -109,6 +109,39 @@ class MPI_particles:
         _lib.check(self.dns.lib.ld_exterp_cosine_scalar(self.dns.ctx, _ptr(coord), npart, _ptr(w), _ptr(c)))
         return c
 
+    # ------------------------------------------------------- collisions (no reference counterpart)
+    def radius(self, nu, rho_p):
+        """Droplet radius r = sqrt(9 nu tau_eta St / (2 rho_p)) (Stokes number of
+        a sphere; the reference's `factor` is not a radius, see its **3/2)."""
+        st = (self.coord[:, -1] / self.factor) ** (2 / 3.0)
+        return torch.sqrt(9 * nu * self.tau_eta * st / (2 * rho_p)).contiguous()
+
+    def collision_pairs(self, radius, max_pairs=1 << 20):
+        """Sorted (i, j) row pairs with minimum-image distance below r_i + r_j."""
+        dns = self.dns
+        coord, npart = dns._rows(self.coord)
+        radius = radius.to(torch.float64).contiguous()
+        rmax = float(radius.max().item()) if npart else 0.0
+        out = torch.empty((max_pairs, 2), dtype=torch.int64, device=dns.device)
+        cnt = ctypes.c_int64()
+        dns._stream()
+        _lib.check(dns.lib.ld_collide_pairs(dns.ctx, _ptr(coord), _ptr(radius), npart, rmax, _ptr(out), max_pairs,
+                                            ctypes.byref(cnt)))
+        return out[: cnt.value].clone()
+
+    def coalesce(self, pairs):
+        """Apply the greedy merge rule in place on coord; returns the (i, j) events."""
+        dns = self.dns
+        coord, npart = dns._rows(self.coord)
+        pairs = pairs.contiguous()
+        ev = torch.empty_like(pairs)
+        ne = ctypes.c_int64()
+        dns._stream()
+        _lib.check(dns.lib.ld_coalesce(dns.ctx, _ptr(coord), npart, _ptr(pairs), pairs.shape[0], _ptr(ev),
+                                       ctypes.byref(ne)))
+        self.update_intrinsic()
+        return ev[: ne.value].clone()
+
     # particles.py:817-837
     def pRHS(self, t, coord, u, us, c, fc, sump):
         """Droplet RHS at the stage rows ``coord``: returns (sump, rhs, fc) and sets

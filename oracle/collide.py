@@ -1,0 +1,72 @@
+"""Brute-force CPU oracle for droplet-droplet collision pairs and coalescence.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+The reference has no droplet-droplet collisions (its droplets grow only by
+collecting the continuum density, particles.py:828).  The specification these
+functions define, and the GPU cell-list path must reproduce bit for bit:
+
+pairs
+    all (i, j), i < j (row indices), with
+        d = x_j - x_i  per axis,  d -= L * rint(d / L)      (minimum image)
+        d2 = (dx*dx + dy*dy) + dz*dz
+        collide  iff  d2 < (r_i + r_j) * (r_i + r_j)
+    in IEEE binary64 with no fused multiply-add, on wrapped positions
+    x mod L; sorted lexicographically by (i, j).
+coalescence
+    pairs are visited in that order; a pair merges only if neither droplet
+    took part in an earlier merge of this call.  The survivor is the smaller
+    index i; with m = m_i + m_j:
+        x_i <- wrap(x_i + (m_j / m) * d)      (d as above, minimum image)
+        v_i <- (m_i v_i + m_j v_j) / m
+        m_i <- m
+    and droplet j is left with m_j = 0 and v_j = 0 (inactive).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def wrap(x, L):
+    r = np.fmod(x, L)
+    return np.where(r < 0, r + L, r)
+
+
+def pairs_bruteforce(pos: np.ndarray, radius: np.ndarray, L: float) -> np.ndarray:
+    """O(Np^2) enumeration; returns an (npairs, 2) int64 array sorted by (i, j)."""
+    x = wrap(np.asarray(pos, dtype=np.float64)[:, :3], L)
+    r = np.asarray(radius, dtype=np.float64)
+    n = x.shape[0]
+    out = []
+    for i in range(n - 1):
+        d = x[i + 1:] - x[i]
+        d = d - L * np.rint(d / L)
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        s = r[i] + r[i + 1:]
+        hit = np.nonzero(d2 < s * s)[0]
+        for k in hit:
+            out.append((i, i + 1 + k))
+    return np.array(out, dtype=np.int64).reshape(-1, 2)
+
+
+def coalesce(rows: np.ndarray, pairs: np.ndarray, L: float):
+    """Apply the greedy merge rule to (Np, 7) rows; returns (rows, events)."""
+    rows = np.array(rows, dtype=np.float64, copy=True)
+    used = np.zeros(rows.shape[0], dtype=bool)
+    events = []
+    for i, j in pairs:
+        if used[i] or used[j]:
+            continue
+        xi = wrap(rows[i, :3], L)
+        xj = wrap(rows[j, :3], L)
+        d = xj - xi
+        d = d - L * np.rint(d / L)
+        mi, mj = rows[i, 6], rows[j, 6]
+        m = mi + mj
+        rows[i, :3] = wrap(xi + (mj / m) * d, L)
+        rows[i, 3:6] = (mi * rows[i, 3:6] + mj * rows[j, 3:6]) / m
+        rows[i, 6] = m
+        rows[j, 3:7] = 0.0
+        used[i] = used[j] = True
+        events.append((i, j))
+    return rows, np.array(events, dtype=np.int64).reshape(-1, 2)

@@ -186,3 +186,90 @@ def test_steps_dense_droplets(gpu, prec):
     assert min_image(a, b, cfg.L) < tol * cfg.L
     assert relerr(a[:, 3:], b[:, 3:]) < tol
     d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_plain_form_matches_oracle(gpu, prec):
+    """The unfused reference-dataflow path (ld_advance_plain) on the same inputs."""
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    nparts = 50
+    cfg = _cfg(32, prec, nparts)
+    uk, n, coord, pid = inputs(cfg, seed=12)
+    fl = make_flow_oracle(cfg)
+    dr = make_drops_oracle(cfg, coord, pid)
+    h = 0.01
+    uk_r, n_r = uk.copy(), n.copy()
+    for _ in range(2):
+        uk_r, n_r = fl.step(0.0, h, dr, uk_r, n_r)
+    d = ForcedDNS(cfg)
+    d.set_state(_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+    d.advance_plain(h, 2)
+    ukg = d.zeros_spec(3)
+    ng = d.zeros_real()
+    cg = torch.zeros((nparts, 7), dtype=torch.float64, device=gpu)
+    d.get_state(ukg, ng, cg)
+    torch.cuda.synchronize()
+    tol = TOL[prec]
+    assert relerr(ukg.cpu().numpy(), uk_r) < tol
+    assert relerr(ng.cpu().numpy(), n_r) < tol
+    a = cg.cpu().numpy()
+    b = by_id(dr.coord, dr.prtclid)
+    assert min_image(a, b, cfg.L) < tol * cfg.L
+    assert relerr(a[:, 3:], b[:, 3:]) < tol
+    d.close()
+
+
+def _collide_case(gpu, n, rlo, rhi, seed, seam=False):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    from lucky_droplets_amd.particles import MPI_particles
+    from oracle import collide
+    cfg = _cfg(32, "f64", n)
+    rng = np.random.default_rng(seed)
+    rows = rng.uniform(0, cfg.L, (n, 7))
+    rows[:, :3] += rng.integers(-2, 3, (n, 3)) * cfg.L        # unwrapped positions
+    if seam:
+        rows[: n // 4, 0] = rng.uniform(-1e-3, 1e-3, n // 4)
+    r = rng.uniform(rlo, rhi, n)
+    d = ForcedDNS(cfg)
+    stb = MPI_particles(None, cfg.L, cfg.N, n, cfg.sts, 0.05, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0,
+                        dns=d, coord=rows)
+    got = stb.collision_pairs(torch.from_numpy(r).to(gpu), max_pairs=1 << 16).cpu().numpy()
+    expect = collide.pairs_bruteforce(rows, r, cfg.L)
+    return d, stb, rows, r, got, expect
+
+
+@pytest.mark.parametrize("n,rlo,rhi,seed", [(3000, 0.005, 0.03, 0), (2000, 0.2, 0.6, 1), (5000, 0.001, 0.02, 2)])
+def test_collision_pairs_bit_exact(gpu, n, rlo, rhi, seed):
+    d, stb, rows, r, got, expect = _collide_case(gpu, n, rlo, rhi, seed, seam=(seed == 2))
+    assert expect.shape[0] > 0
+    assert np.array_equal(got, expect)
+    d.close()
+
+
+def test_coalescence_bit_exact(gpu):
+    from oracle import collide
+    d, stb, rows, r, pairs, expect = _collide_case(gpu, 2000, 0.01, 0.05, 3)
+    assert np.array_equal(pairs, expect) and pairs.shape[0] > 10
+    import torch
+    ev = stb.coalesce(torch.from_numpy(pairs).to(gpu)).cpu().numpy()
+    ref_rows, ref_ev = collide.coalesce(rows, expect, stb.L)
+    assert np.array_equal(ev, ref_ev)
+    assert np.array_equal(stb.coord.cpu().numpy(), ref_rows)
+    d.close()
+
+
+def test_collision_capacity_error(gpu):
+    import torch
+    from lucky_droplets_amd import _lib
+    from lucky_droplets_amd.dns import ForcedDNS
+    from lucky_droplets_amd.particles import MPI_particles
+    cfg = _cfg(32, "f64", 500)
+    rows = np.random.default_rng(5).uniform(0, cfg.L, (500, 7))
+    d = ForcedDNS(cfg)
+    stb = MPI_particles(None, cfg.L, cfg.N, 500, cfg.sts, 0.05, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0,
+                        dns=d, coord=rows)
+    with pytest.raises(_lib.LdError):
+        stb.collision_pairs(torch.full((500,), 0.5, dtype=torch.float64, device=gpu), max_pairs=4)
+    d.close()
