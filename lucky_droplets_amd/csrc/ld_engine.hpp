@@ -71,7 +71,9 @@ struct Engine final : EngineBase {
   hipStream_t st2 = nullptr;        // density/droplet stream (created here)
   hipEvent_t evA = nullptr, evB = nullptr;
   C<T>*uk = nullptr, *us = nullptr, *acc = nullptr, *w6 = nullptr, *wn = nullptr;
-  T* ubuf[2] = {nullptr, nullptr};
+  T* ubuf[2] = {nullptr, nullptr};       // planar physical u (API / plain paths)
+  V4<T>* gu[2] = {nullptr, nullptr};     // node-interleaved {u, 0}, one per stage parity
+  V4<T>* gv = nullptr;                   // node-interleaved {v, n_stage}
   T *v = nullptr, *n_state = nullptr, *n_stage = nullptr, *nacc = nullptr, *fc = nullptr;
   double *partials = nullptr, *partials2 = nullptr, *clip_stage = nullptr, *clip_end = nullptr, *clip_none = nullptr;
   double *maxpart = nullptr, *ek = nullptr, *fac = nullptr, *lastmax = nullptr;
@@ -86,7 +88,10 @@ struct Engine final : EngineBase {
   int *pkey = nullptr, *prank = nullptr, *hist = nullptr, *bsum = nullptr;
   double* x0b = nullptr;
   PT* v0b = nullptr;
+  double4* dep = nullptr;            // per-slot stage position and W for the brick deposit
+  int *flist = nullptr, *fcount = nullptr;
   bool sort_enabled = true;
+  bool sorted_ok = false;            // bin starts (hist) describe the current slots
   static constexpr int NB = Bins<N>::NB;
   static constexpr int NBS = (NB + 4095) / 4096;
   DropPhys dp;
@@ -140,6 +145,7 @@ struct Engine final : EngineBase {
     LD_TRY(alloc(uk, 3 * CF)); LD_TRY(alloc(us, 3 * CF)); LD_TRY(alloc(acc, 3 * CF)); LD_TRY(alloc(w6, 6 * CF));
     LD_TRY(alloc(wn, 4 * CF));
     LD_TRY(alloc(ubuf[0], 3 * RF)); LD_TRY(alloc(ubuf[1], 3 * RF)); LD_TRY(alloc(v, 3 * RF));
+    LD_TRY(alloc(gu[0], RF)); LD_TRY(alloc(gu[1], RF)); LD_TRY(alloc(gv, RF));
     LD_TRY(alloc(n_state, RF)); LD_TRY(alloc(n_stage, RF)); LD_TRY(alloc(nacc, RF)); LD_TRY(alloc(fc, RF));
     const int npart = NBLK_Z1 > NBLK_SUM ? NBLK_Z1 : NBLK_SUM;
     LD_TRY(alloc(partials, 2 * (size_t)npart));
@@ -162,6 +168,7 @@ struct Engine final : EngineBase {
       LD_TRY(alloc(rid, cap)); LD_TRY(alloc(rid2, cap)); LD_TRY(alloc(pkey, cap)); LD_TRY(alloc(prank, cap));
       LD_TRY(alloc(hist, NB)); LD_TRY(alloc(bsum, NBS));
       LD_TRY(alloc(x0b, 3 * cap)); LD_TRY(alloc(v0b, 4 * cap));
+      LD_TRY(alloc(dep, cap)); LD_TRY(alloc(flist, cap)); LD_TRY(alloc(fcount, 1));
     }
     static_assert(NBS <= 4096, "bin-start scan supports at most 4096 partial sums");
     dp.L = p.L;
@@ -301,13 +308,13 @@ struct Engine final : EngineBase {
   // ------------------------------------------------------------ stage pieces
   // Flow chain on stream s: u, omega -> physical u (uphys), z/y-forward of
   // u x omega in w6[0..3).
-  int nonlinear(hipStream_t s, const C<T>* usp, T* uphys) {
+  int nonlinear(hipStream_t s, const C<T>* usp, T* uphys, V4<T>* g = nullptr) {
     LD_TRY(prof_mark(LD_K_XUW, s));
     hipLaunchKernelGGL((k_x_uw<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, s, usp, w6, CF, (const C<T>*)tw, inv());
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XUW, s));
     LD_TRY((pass<1, +1>(s, w6, w6, 6, inv())));
-    OpZCross<T, N, P6> oc{w6, CF, uphys, inv()};
+    OpZCross<T, N, P6> oc{w6, CF, uphys, g, inv()};
     LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS)));
     LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1)));
     return LD_OK;
@@ -315,8 +322,8 @@ struct Engine final : EngineBase {
   // v_hat (x-inverse in w6[0..3)): y-inverse; then z-pass to physical v, and n v
   // z-forward into wn[0..3).
   int velocity_y(hipStream_t s) { return pass<1, +1>(s, w6, w6, 3, inv()); }
-  int velocity_z(hipStream_t s, T* vphys, const T* ns, const double* clip) {
-    OpZVel<T, N, P3> ov{w6, wn, CF, vphys, ns, clip, inv(), (T)(p.tps * p.g)};
+  int velocity_z(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip) {
+    OpZVel<T, N, P3> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
     return zrun<P3, 3, 3>(s, ov, LD_K_ZVEL);
   }
   // Density chain on stream s: fc -> z-forward into wn[3]; y-forward of
@@ -345,11 +352,11 @@ struct Engine final : EngineBase {
     static const double a_coef[5] = {0.0, 0.5, 0.5, 1.0, 0.0};
     const C<T>* usp = (s == 0) ? uk : us;
     const T* ns = (s == 0) ? n_state : n_stage;
-    T* uphys = ubuf[s & 1];
+    V4<T>* gup = gu[s & 1];
     const double wh = (s == 0 || s == 3) ? h / 6.0 : h / 3.0;   // forced-dns-sm.py:342-358
     const double anext = a_coef[s + 1] * h;
     LD_TRY(shell(st, usp, p.isforcing));
-    LD_TRY(nonlinear(st, usp, uphys));
+    LD_TRY(nonlinear(st, usp, nullptr, gup));
     RhsArgs<T> ra;
     ra.nl = w6; ra.us = usp; ra.ubase = uk; ra.acc = acc; ra.usnext = (s < 3) ? us : nullptr; ra.ku = nullptr;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
@@ -361,14 +368,16 @@ struct Engine final : EngineBase {
     LD_TRY(velocity_y(st));
     // v, nv need: the previous density chain done (n_s, clip, wn and v free)
     LD_HIP(hipStreamWaitEvent(st, evB, 0));
-    LD_TRY(velocity_z(st, v, ns, clip_stage));
+    LD_TRY(velocity_z(st, nullptr, gv, ns, clip_stage));
     LD_TRY(join(st2, st, evA));
-    LD_TRY(fill<T>(st2, fc, RF, (T)0));
+    if (!sorted_ok || np == 0) LD_TRY(fill<T>(st2, fc, RF, (T)0));
     if (np > 0) {
+      LD_TRY(fill<int>(st2, fcount, 1, 0));
       PStage<T, PT> ps;
       ps.np = (int)np; ps.cap = (int)cap; ps.stage = s;
       ps.x0 = x0; ps.v0 = v0; ps.sx = sx; ps.sv = sv; ps.kx = kxp; ps.kv = kvp;
-      ps.u = uphys; ps.v = v; ps.n = ns; ps.clip = clip_stage; ps.fc = fc;
+      ps.gu = gup; ps.gv = gv; ps.dep = dep; ps.flist = flist; ps.fcount = fcount;
+      ps.all_fallback = sorted_ok ? 0 : 1;
       ps.ah = a_coef[s] * h; ps.wh = wh; ps.P = dp;
       // 4 lanes per droplet; block count a multiple of 8 for the XCD mapping
       const unsigned nb = (unsigned)(((4 * np + 255) / 256 + 7) / 8 * 8);
@@ -376,6 +385,15 @@ struct Engine final : EngineBase {
       hipLaunchKernelGGL((k_particles_stage<T, PT, N>), dim3(nb), dim3(256), 0, st2, ps);
       LD_LAUNCHED();
       LD_TRY(prof_mark(LD_K_DROPLETS, st2));
+      if (sorted_ok) {
+        hipLaunchKernelGGL((k_brick_deposit<T, N>), dim3(Brick<N>::NB * Brick<N>::NB * Brick<N>::NB), dim3(256), 0,
+                           st2, (const double4*)dep, (const int*)hist, (int)np, fc, dp);
+        LD_LAUNCHED();
+      }
+      const unsigned fb = sorted_ok ? 64u : (unsigned)((np + 255) / 256);
+      hipLaunchKernelGGL((k_fallback_deposit<T, N>), dim3(fb), dim3(256), 0, st2, (const double4*)dep,
+                         (const int*)flist, (const int*)fcount, fc, dp);
+      LD_LAUNCHED();
     }
     LD_TRY(density_rhs(st2, fc));
     OpZKn<T, N, P1> ok;
@@ -427,6 +445,7 @@ struct Engine final : EngineBase {
 
   // ------------------------------------------------------------ state
   int set_rows(const double* coord, int64_t n_) {
+    sorted_ok = false;
     if (n_ > cap) { set_error("%lld droplets exceed the context capacity %lld", (long long)n_, (long long)cap); return LD_ERR_CAPACITY; }
     np = n_;
     if (np > 0) {
@@ -463,7 +482,8 @@ struct Engine final : EngineBase {
   // Counting sort of the droplet slots by spatial bin of the base position
   // (on the density stream, which owns the droplet state during a step).
   int sort_droplets(hipStream_t s) {
-    if (np < 2 || !sort_enabled) return LD_OK;
+    sorted_ok = false;
+    if (np < 1 || !sort_enabled) return LD_OK;
     const unsigned pb = (unsigned)((np + 255) / 256);
     LD_TRY(fill<int>(s, hist, NB, 0));
     hipLaunchKernelGGL((k_bin_count<N>), dim3(pb), dim3(256), 0, s, (const double*)x0, (int)np, (int)cap, p.L,
@@ -482,6 +502,7 @@ struct Engine final : EngineBase {
     std::swap(x0, x0b);
     std::swap(v0, v0b);
     std::swap(rid, rid2);
+    sorted_ok = true;
     return LD_OK;
   }
 
@@ -540,7 +561,7 @@ struct Engine final : EngineBase {
     LD_LAUNCHED();
     // the physical-space v the droplets see uses visc as well (forced-dns-sm.py:319-321)
     LD_TRY(velocity_y(st));
-    LD_TRY(velocity_z(st, vphys, (const T*)nin, clip_none));
+    LD_TRY(velocity_z(st, vphys, nullptr, (const T*)nin, clip_none));
     LD_TRY(fill<T>(st, fcp, RF, (T)0));
     if (n_ > 0) {
       hipLaunchKernelGGL((k_prhs<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, coord, (int)n_,

@@ -19,6 +19,10 @@ template <int N> struct Geo {
   static constexpr size_t R = (size_t)N * N * N;    // reals per field
 };
 
+// Node-interleaved 4-vector of the physical fields the droplets gather
+// ({u_x, u_y, u_z, 0} and {v_x, v_y, v_z, n_stage}).
+template <typename T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
+
 // Loads/stores at a 32-bit element index from a uniform base pointer: the
 // byte offset stays 32-bit so the address is SGPR base + VGPR offset.
 template <typename X>
@@ -305,7 +309,7 @@ struct OpZRound {
 // 3 fields written over inputs 0..2.
 template <typename T, int N, int P>
 struct OpZCross {
-  C<T>* w; size_t fs; T* u; T scale;
+  C<T>* w; size_t fs; T* u; V4<T>* gu; T scale;   // u planar (may be null) and/or node-interleaved (may be null)
   __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
@@ -314,7 +318,8 @@ struct OpZCross {
       const T ux = t.re(0, p, j) * scale, uy = t.re(1, p, j) * scale, uz = t.re(2, p, j) * scale;
       const T wx = t.re(3, p, j) * scale, wy = t.re(4, p, j) * scale, wz = t.re(5, p, j) * scale;
       const size_t o = (size_t)(line0 + p) * N + j;
-      u[o] = ux; u[R + o] = uy; u[2 * R + o] = uz;
+      if (u) { u[o] = ux; u[R + o] = uy; u[2 * R + o] = uz; }
+      if (gu) { V4<T> q; q.x = ux; q.y = uy; q.z = uz; q.w = (T)0; gu[o] = q; }
       t.re(0, p, j) = wz * uy - wy * uz;
       t.re(1, p, j) = wx * uz - wz * ux;
       t.re(2, p, j) = wy * ux - wx * uy;
@@ -340,7 +345,7 @@ __device__ __forceinline__ T nStage(T x, const double* __restrict__ clip) {
 // pass), v stored, then n*v transformed along z (:326-329, first pass).
 template <typename T, int N, int P>
 struct OpZVel {
-  const C<T>* w; C<T>* wout; size_t fs; T* v; const T* n; const double* clip; T scale; T tpsg;
+  const C<T>* w; C<T>* wout; size_t fs; T* v; V4<T>* gv; const T* n; const double* clip; T scale; T tpsg;
   __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
@@ -349,8 +354,9 @@ struct OpZVel {
       const size_t o = (size_t)(line0 + p) * N + j;
       const T vx = t.re(0, p, j) * scale, vy = t.re(1, p, j) * scale;
       const T vz = t.re(2, p, j) * scale - tpsg;
-      v[o] = vx; v[R + o] = vy; v[2 * R + o] = vz;
       const T nn = nStage<T>(n[o], clip);
+      if (v) { v[o] = vx; v[R + o] = vy; v[2 * R + o] = vz; }
+      if (gv) { V4<T> q; q.x = vx; q.y = vy; q.z = vz; q.w = nn; gv[o] = q; }
       t.re(0, p, j) = nn * vx;
       t.re(1, p, j) = nn * vy;
       t.re(2, p, j) = nn * vz;

@@ -156,14 +156,26 @@ __device__ __forceinline__ void dropletRhs(const DropPhys& P, const double vp[3]
   rhs[6] = W;                                                               // :835
 }
 
+// ------------------------------------------------------------ spatial sort
+// Droplets are reordered by spatial bin once per step (counting sort), so
+// that lanes of a wavefront gather from and deposit into the same cache
+// lines.  Bins are cubes of BS = max(2, N/128) grid cells.
+template <int N> struct Bins {
+  static constexpr int BS = (N / 128 > 2) ? N / 128 : 2;
+  static constexpr int B = N / BS;
+  static constexpr int NB = B * B * B;
+};
+
 // ------------------------------------------------------------ fused RK stage
 template <typename T, typename PT> struct PStage {
   int np, cap, stage;          // stage 0..3
   double* x0; PT* v0;          // RK base (unwrapped); final stage writes the new state here
   double* sx; PT* sv;          // sump accumulator
   double* kx; PT* kv;          // stage rhs (read: previous stage, write: this stage)
-  const T* u; const T* v; const T* n; const double* clip;
-  T* fc;
+  const V4<T>* gu; const V4<T>* gv;   // {u, 0} and {v, n_stage} per node
+  double4* dep;                // out: wrapped stage position and W per slot (brick deposit input)
+  int* flist; int* fcount;     // out: slots whose stage bin moved > 1 bin since the sort
+  int all_fallback;            // 1: slots are not bin-sorted, every slot takes the fallback deposit
   double ah, wh;               // a_s * h, w_s * h
   DropPhys P;
 };
@@ -213,11 +225,10 @@ k_particles_stage(PStage<T, PT> A) {
   for (int b = 0; b < 4; ++b) { wy[b] = cosW(yw, j0 + b, dx, pi_2dx); iy[b] = wrapIdx<N>(j0 + b); }
   const double wz = cosW(zw, k0 + c, dx, pi_2dx);
   const int iz = wrapIdx<N>(k0 + c);
-  // gather: this lane's z node, all (x, y) nodes
+  // gather: this lane's z node, all (x, y) nodes; two 16-byte records per node
   double I[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) I[k] = 0.0;
-  const bool doclip = A.clip[2] != 0.0;
   if (live) {
 #pragma unroll 1
     for (int a = 0; a < 4; ++a) {
@@ -230,15 +241,15 @@ k_particles_stage(PStage<T, PT> A) {
       for (int b = 0; b < 4; ++b) {
         const size_t o = (plane + iy[b]) * N + iz;
         const double w = wy[b];
-        t[0] += (double)A.u[o] * w;
-        t[1] += (double)A.u[R + o] * w;
-        t[2] += (double)A.u[2 * R + o] * w;
-        t[3] += (double)A.v[o] * w;
-        t[4] += (double)A.v[R + o] * w;
-        t[5] += (double)A.v[2 * R + o] * w;
-        double nv = (double)A.n[o];
-        if (doclip && nv < 0.0) nv = 0.0;
-        t[6] += nv * w;
+        const V4<T> U = A.gu[o];
+        const V4<T> V = A.gv[o];
+        t[0] += (double)U.x * w;
+        t[1] += (double)U.y * w;
+        t[2] += (double)U.z * w;
+        t[3] += (double)V.x * w;
+        t[4] += (double)V.y * w;
+        t[5] += (double)V.z * w;
+        t[6] += (double)V.w * w;
       }
 #pragma unroll
       for (int k = 0; k < 7; ++k) I[k] += t[k] * wx;
@@ -252,21 +263,27 @@ k_particles_stage(PStage<T, PT> A) {
     I[k] = v;
   }
   if (!live) return;
-  if (doclip) I[6] = (I[6] * A.clip[0]) / A.clip[1];
   double rhs[7], W;
   dropletRhs(A.P, vp, m, I, rhs, W);
-  // deposit of W with the same kernel / dx^3 (particles.py:23-35, 529-575)
-  {
-    const double wzv = W * wz * A.P.inv_vol;
-#pragma unroll 1
-    for (int a = 0; a < 4; ++a) {
-      const double wa = wzv * cosW(xw, i0 + a, dx, pi_2dx);
-      const size_t plane = (size_t)wrapIdx<N>(i0 + a) * N;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) unsafeAtomicAdd(&A.fc[(plane + iy[b]) * N + iz], (T)(wa * wy[b]));
-    }
-  }
   if (c != 0) return;
+  // deposit input for the brick pass (k_brick_deposit); slots whose stage
+  // bin left the +-1-bin window around their sorted bin go to the fallback list
+  {
+    bool far = A.all_fallback != 0;
+    const double xb[3] = {xw, yw, zw};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      // bins exactly as k_bin_count forms them
+      int cs = (int)(xb[k] / dx), c0 = (int)(wrapL(A.x0[k * cap + p], L) / dx);
+      cs = cs < 0 ? 0 : (cs > N - 1 ? N - 1 : cs);
+      c0 = c0 < 0 ? 0 : (c0 > N - 1 ? N - 1 : c0);
+      const int d = ((cs / Bins<N>::BS - c0 / Bins<N>::BS) % Bins<N>::B + Bins<N>::B) % Bins<N>::B;
+      far = far || (d > 1 && d < Bins<N>::B - 1);
+    }
+    // a fallback slot stores -(W + 1) so the brick pass skips it
+    A.dep[p] = make_double4(xw, yw, zw, far ? -(W + 1.0) : W);
+    if (far) A.flist[atomicAdd(A.fcount, 1)] = p;
+  }
   // sump (forced-dns-sm.py:337, 342-358); the last stage writes the new state
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -285,6 +302,97 @@ k_particles_stage(PStage<T, PT> A) {
     for (int k = 0; k < 3; ++k) A.kx[k * cap + p] = rhs[k];
 #pragma unroll
     for (int k = 0; k < 4; ++k) A.kv[k * cap + p] = (PT)rhs[3 + k];
+  }
+}
+
+// ------------------------------------------------------------ brick deposit
+// Deposit of W with the cosine kernel / dx^3 (particles.py:23-35, 529-575),
+// organised by output: one workgroup per brick of 8^3 grid nodes scans the
+// sorted-bin ranges that can reach the brick (stencil -1..+2 cells, plus one
+// bin of motion since the sort), accumulates in LDS and writes each node of
+// the brick exactly once.  Slots on the fallback list are skipped here.
+template <int N> struct Brick {
+  static constexpr int S = 8;                       // nodes per side
+  static constexpr int NB = N / S;                  // bricks per side
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, int np, T* __restrict__ fc,
+                DropPhys P) {
+  constexpr int S = Brick<N>::S, NBr = Brick<N>::NB;
+  constexpr int BS = Bins<N>::BS, B = Bins<N>::B;
+  __shared__ float tile_f[(sizeof(T) == 4) ? S * S * S : 1];
+  __shared__ double tile_d[(sizeof(T) == 8) ? S * S * S : 1];
+  T* tile = (sizeof(T) == 4) ? (T*)tile_f : (T*)tile_d;
+  const int br = blockIdx.x;
+  const int c0x = (br / (NBr * NBr)) * S, c0y = ((br / NBr) % NBr) * S, c0z = (br % NBr) * S;
+  for (int i = threadIdx.x; i < S * S * S; i += blockDim.x) tile[i] = (T)0;
+  __syncthreads();
+  // bins whose droplets can reach the brick: stage cells [c0-2, c0+S] -> bins, +-1 bin of motion
+  const int blx = (c0x - 2 + N) % N / BS, bly = (c0y - 2 + N) % N / BS, blz = (c0z - 2 + N) % N / BS;
+  // stage cells c0-2 .. c0+S span at most (S + 2 + BS - 1)/BS + 1 bins; one more on each side for motion
+  int nbx = (S + 2 + BS - 1) / BS + 1 + 2;
+  if (nbx > B) nbx = B;
+  const int lo_shift = (nbx == B) ? 0 : 1;          // start one bin below the stencil's lowest bin
+  const int nruns = nbx * nbx;
+  const double dx = P.dx, pi_2dx = P.pi_2dx;
+  for (int r = threadIdx.x >> 2; r < nruns; r += blockDim.x >> 2) {
+    const int bx = (blx - lo_shift + r / nbx + B) % B, by = (bly - lo_shift + r % nbx + B) % B;
+    for (int q = 0; q < nbx; ++q) {
+      const int bz = (blz - lo_shift + q + B) % B;
+      const int bin = (bx * B + by) * B + bz;
+      const int b = start[bin], e = (bin + 1 < B * B * B) ? start[bin + 1] : np;
+      for (int t = b + (threadIdx.x & 3); t < e; t += 4) {
+        const double4 d = dep[t];
+        if (!(d.w > 0.0)) continue;             // no growth, or a fallback slot
+        const int i0 = (int)floor(d.x / dx) - 1, j0 = (int)floor(d.y / dx) - 1, k0 = (int)floor(d.z / dx) - 1;
+        // brick-relative first node of the stencil (periodic); reject without overlap
+        const int rx = ((i0 - c0x) % N + N) % N, ry = ((j0 - c0y) % N + N) % N, rz = ((k0 - c0z) % N + N) % N;
+        const bool ox = rx < S || rx > N - 4, oy = ry < S || ry > N - 4, oz = rz < S || rz > N - 4;
+        if (!(ox && oy && oz)) continue;
+        const double W = d.w * P.inv_vol;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int nx = (rx + a) % N;
+          if (nx >= S) continue;
+          const double wa = W * cosW(d.x, i0 + a, dx, pi_2dx);
+#pragma unroll
+          for (int bb = 0; bb < 4; ++bb) {
+            const int ny = (ry + bb) % N;
+            if (ny >= S) continue;
+            const double wab = wa * cosW(d.y, j0 + bb, dx, pi_2dx);
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+              const int nz = (rz + cc) % N;
+              if (nz >= S) continue;
+              atomicAdd(&tile[(nx * S + ny) * S + nz], (T)(wab * cosW(d.z, k0 + cc, dx, pi_2dx)));
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < S * S * S; i += blockDim.x) {
+    const int nx = i / (S * S), ny = (i / S) % S, nz = i % S;
+    fc[((size_t)(c0x + nx) * N + (c0y + ny)) * N + (c0z + nz)] = tile[i];
+  }
+}
+
+// Fallback deposit (global atomics) of the slots that moved more than one
+// bin since the sort; runs after k_brick_deposit.
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_fallback_deposit(const double4* __restrict__ dep, const int* __restrict__ flist, const int* __restrict__ fcount,
+                   T* fc, DropPhys P) {
+  const int nf = *fcount;
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+    const double4 d = dep[flist[f]];
+    const double xw[3] = {d.x, d.y, d.z};
+    Stencil<N> s;
+    s.set(xw, P.dx, P.pi_2dx);
+    deposit<T, N>(s, -d.w - 1.0, P.inv_vol, fc);
   }
 }
 
@@ -372,16 +480,6 @@ static __global__ void k_axpy_rows(const double* __restrict__ x, const double* _
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = x[i] + a * k[i];
 }
-
-// ------------------------------------------------------------ spatial sort
-// Droplets are reordered by spatial bin once per step (counting sort), so
-// that lanes of a wavefront gather from and deposit into the same cache
-// lines.  Bins are cubes of BS = max(2, N/128) grid cells.
-template <int N> struct Bins {
-  static constexpr int BS = (N / 128 > 2) ? N / 128 : 2;
-  static constexpr int B = N / BS;
-  static constexpr int NB = B * B * B;
-};
 
 template <int N>
 __global__ void __launch_bounds__(256)

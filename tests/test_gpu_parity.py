@@ -250,7 +250,7 @@ def test_collision_pairs_bit_exact(gpu, n, rlo, rhi, seed):
 
 def test_coalescence_bit_exact(gpu):
     from oracle import collide
-    d, stb, rows, r, pairs, expect = _collide_case(gpu, 2000, 0.01, 0.05, 3)
+    d, stb, rows, r, pairs, expect = _collide_case(gpu, 2000, 0.05, 0.15, 3)
     assert np.array_equal(pairs, expect) and pairs.shape[0] > 10
     import torch
     ev = stb.coalesce(torch.from_numpy(pairs).to(gpu)).cpu().numpy()
