@@ -149,6 +149,19 @@ int ld_exterp_cosine_scalar(ld_ctx* ctx, const double* coord, int64_t np, const 
 int ld_prhs(ld_ctx* ctx, const double* coord, int64_t np, const void* u, const void* us, const void* n,
             void* fc, double* rhs, double* interpmat, double* exterpmat);
 
+/* --------------------------------------------------------- diagnostics */
+/* Shell spectrum e3d_to_e1d(weight * sum_c |f_c|^2 * normalize) of nfields
+ * spectral fields into out[N/2+1] (device double): weight 0.5 on u_hat gives
+ * E(k) (forced-dns-sm.py:423, 431); weight 1 on n_hat the n spectrum (plot.py:37-38). */
+int ld_spectrum(ld_ctx* ctx, const void* f, int32_t nfields, double weight, double* out);
+/* Energy flux Pi(k): e3d_to_e1d(Re(conj(uk) . ku) * dealias * normalize),
+ * cumulated from the highest shell down (forced-dns-sm.py:426-428); ku from
+ * ld_full_rhs with visc = 0, forc = 0 (:425).  out[N/2+1] device double. */
+int ld_flux(ld_ctx* ctx, const void* uk, const void* ku, double* out);
+/* sum, min, max and sum of squares over nfields real fields -> out[4]
+ * (device double): energy, n mean/min/max (forced-dns-sm.py:467-471). */
+int ld_field_stats(ld_ctx* ctx, const void* x, int32_t nfields, double* out);
+
 /* ---------------------------------------------------------- collisions */
 /* Droplet-droplet collision pairs with a cell list.  No counterpart in the
  * reference (its droplets grow only by collecting the continuum density,

@@ -196,6 +196,23 @@ int ld_profile_read(ld_ctx* ctx, double* total_ms, int64_t* launches) {
   return ctx->eng->profile_read(total_ms, launches);
 }
 
+int ld_spectrum(ld_ctx* ctx, const void* f, int32_t nfields, double weight, double* out) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(f); LD_NONNULL(out);
+  if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
+  return ctx->eng->spectrum(f, nfields, weight, out);
+}
+
+int ld_flux(ld_ctx* ctx, const void* uk, const void* ku, double* out) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(ku); LD_NONNULL(out);
+  return ctx->eng->flux(uk, ku, out);
+}
+
+int ld_field_stats(ld_ctx* ctx, const void* x, int32_t nfields, double* out) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(x); LD_NONNULL(out);
+  if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
+  return ctx->eng->stats(x, nfields, out);
+}
+
 int ld_collide_pairs(ld_ctx* ctx, const double* rows, const double* radius, int64_t np, double rmax,
                      int64_t* pairs, int64_t max_pairs, int64_t* npairs) {
   LD_CHECK_CTX(ctx); LD_NONNULL(npairs);

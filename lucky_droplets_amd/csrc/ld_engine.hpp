@@ -5,7 +5,9 @@
 #include "ld_particles.hpp"
 #include "ld_plain.hpp"
 #include "ld_collide.hpp"
+#include "ld_diag.hpp"
 #include <cmath>
+#include <cstdlib>
 #include <utility>
 #include <vector>
 
@@ -91,6 +93,9 @@ struct Engine final : EngineBase {
   double4* dep = nullptr;            // per-slot stage position and W for the brick deposit
   int *flist = nullptr, *fcount = nullptr;
   bool sort_enabled = true;
+  // y and z passes of a chain run slab by slab (x-planes) so the y<->z
+  // intermediate of one slab stays in the Infinity Cache; 1 = whole field.
+  int nslab = 1;
   bool sorted_ok = false;            // bin starts (hist) describe the current slots
   static constexpr int NB = Bins<N>::NB;
   static constexpr int NBS = (NB + 4095) / 4096;
@@ -131,6 +136,10 @@ struct Engine final : EngineBase {
     }
     if (p.max_particles < 0 || p.max_particles > (int64_t)0x7fffffff) { set_error("max_particles out of range"); return LD_ERR_ARG; }
     cap = p.max_particles;
+    if (const char* e = getenv("LD_SLABS")) {
+      const int v = atoi(e);
+      if (v >= 1 && N % v == 0 && (N / v) % M == 0) nslab = v;
+    }
     // twiddles exp(-2 pi i m / N), rounded from long double
     std::vector<C<T>> h(N);
     for (int m = 0; m < N; ++m) {
@@ -182,19 +191,25 @@ struct Engine final : EngineBase {
   void set_stream(hipStream_t s) override { st = s; }
 
   // ------------------------------------------------------------ launch helpers
-  template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale) {
+  // Line pass over x-planes [x0, x0 + nx) (AXIS 1) or all lines (AXIS 0).
+  template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale, int x0 = 0,
+                                        int nx = N) {
     const int kid = AXIS == 1 ? LD_K_YPASS : -2;
+    const int nblk = AXIS == 1 ? nx * Nf / M : NBLK_X;
+    const int qoff = AXIS == 1 ? x0 * Nf : 0;
     LD_TRY(prof_mark(kid, s));
-    hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(NBLK_X, nf), dim3(M * N1), 0, s, in, out, CF,
-                       (const C<T>*)tw, scale);
+    hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(nblk, nf), dim3(M * N1), 0, s, in, out, CF,
+                       (const C<T>*)tw, scale, qoff);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
   }
-  template <int P, int FIN, int FOUT, class Op> int zrun(hipStream_t s, const Op& op, int kid = -2) {
+  // z pass over the (x, y) lines of x-planes [x0, x0 + nx).
+  template <int P, int FIN, int FOUT, class Op> int zrun(hipStream_t s, const Op& op, int kid = -2, int x0 = 0,
+                                                         int nx = N) {
     LD_TRY(prof_mark(kid, s));
-    hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(N * N / P), dim3(ZThreads<N, P, FIN, FOUT>::v), 0, s, op,
-                       (const C<T>*)tw);
+    hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(nx * N / P), dim3(ZThreads<N, P, FIN, FOUT>::v), 0, s, op,
+                       (const C<T>*)tw, x0 * N);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
@@ -313,30 +328,48 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL((k_x_uw<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, s, usp, w6, CF, (const C<T>*)tw, inv());
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XUW, s));
-    LD_TRY((pass<1, +1>(s, w6, w6, 6, inv())));
     OpZCross<T, N, P6> oc{w6, CF, uphys, g, inv()};
-    LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS)));
-    LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1)));
+    const int nx = N / nslab;
+    for (int x0 = 0; x0 < N; x0 += nx) {
+      LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx)));
+      LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx)));
+      LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx)));
+    }
     return LD_OK;
   }
   // v_hat (x-inverse in w6[0..3)): y-inverse; then z-pass to physical v, and n v
   // z-forward into wn[0..3).
-  int velocity_y(hipStream_t s) { return pass<1, +1>(s, w6, w6, 3, inv()); }
-  int velocity_z(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip) {
+  // v_hat y-inverse and z pass (v out, n v forward) slab by slab; `wait`
+  // (may be null) is waited for before the first z pass writes v / nv.
+  int velocity(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip, hipEvent_t wait) {
     OpZVel<T, N, P3> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
-    return zrun<P3, 3, 3>(s, ov, LD_K_ZVEL);
+    const int nx = N / nslab;
+    for (int x0 = 0; x0 < N; x0 += nx) {
+      LD_TRY((pass<1, +1>(s, w6, w6, 3, inv(), x0, nx)));
+      if (x0 == 0 && wait) LD_HIP(hipStreamWaitEvent(s, wait, 0));
+      LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx)));
+    }
+    return LD_OK;
   }
   // Density chain on stream s: fc -> z-forward into wn[3]; y-forward of
   // wn[0..4); kn_hat x-inverse in wn[0]; y-inverse.
-  int density_rhs(hipStream_t s, const T* fcin) {
+  // (leaves kn_hat y-inverse-transformed in wn[0] when kz == nullptr; with
+  // kz the z pass runs per slab too)
+  template <class OpKn> int density_rhs(hipStream_t s, const T* fcin, const OpKn* kz) {
     OpZFwd<T, N, P1, 1> of{fcin, wn + 3 * CF, RF, CF};
-    LD_TRY((zrun<P1, 0, 1>(s, of)));
-    LD_TRY((pass<1, -1>(s, wn, wn, 4, (T)1)));
+    const int nx = N / nslab;
+    for (int x0 = 0; x0 < N; x0 += nx) {
+      LD_TRY((zrun<P1, 0, 1>(s, of, -2, x0, nx)));
+      LD_TRY((pass<1, -1>(s, wn, wn, 4, (T)1, x0, nx)));
+    }
     LD_TRY(prof_mark(LD_K_XKN, s));
     hipLaunchKernelGGL((k_x_kn<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, s, wn, CF, (const C<T>*)tw, inv());
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XKN, s));
-    LD_TRY((pass<1, +1>(s, wn, wn, 1, inv())));
+    for (int x0 = 0; x0 < N; x0 += nx) {
+      LD_TRY((pass<1, +1>(s, wn, wn, 1, inv(), x0, nx)));
+      if (kz) LD_TRY((zrun<P1, 1, 0>(s, *kz, LD_K_ZKN, x0, nx)));
+    }
     return LD_OK;
   }
 
@@ -365,10 +398,8 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL((k_x_rhs<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, ra, (const C<T>*)tw);
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XRHS, st));
-    LD_TRY(velocity_y(st));
-    // v, nv need: the previous density chain done (n_s, clip, wn and v free)
-    LD_HIP(hipStreamWaitEvent(st, evB, 0));
-    LD_TRY(velocity_z(st, nullptr, gv, ns, clip_stage));
+    // v, nv need the previous density chain done (n_s, clip, wn and v free)
+    LD_TRY(velocity(st, nullptr, gv, ns, clip_stage, evB));
     LD_TRY(join(st2, st, evA));
     if (!sorted_ok || np == 0) LD_TRY(fill<T>(st2, fc, RF, (T)0));
     if (np > 0) {
@@ -395,13 +426,12 @@ struct Engine final : EngineBase {
                          (const int*)flist, (const int*)fcount, fc, dp);
       LD_LAUNCHED();
     }
-    LD_TRY(density_rhs(st2, fc));
     OpZKn<T, N, P1> ok;
     ok.w = wn; ok.fs = CF; ok.scale = inv(); ok.kn_out = nullptr;
     ok.n0 = n_state; ok.nacc = nacc; ok.nnext = (s < 3) ? n_stage : nullptr;
     ok.wacc = (T)wh; ok.anext = (T)anext; ok.acc_init = (s == 0);
     ok.sum_target = (s < 3) ? 0 : 1; ok.partials = partials2;
-    LD_TRY((zrun<P1, 1, 0>(st2, ok, LD_K_ZKN)));
+    LD_TRY(density_rhs(st2, fc, &ok));
     hipLaunchKernelGGL(k_clip_final, dim3(1), dim3(256), 0, st2, (const double*)partials2, NBLK_Z1,
                        1.0 / ((double)N * N * N), (s < 3) ? clip_stage : clip_end);
     LD_LAUNCHED();
@@ -430,10 +460,13 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st2, nacc, n_state, RF, (const double*)clip_end, 1);
     LD_LAUNCHED();
     LD_TRY((pass<0, +1>(st, acc, w6, 3, inv())));
-    LD_TRY((pass<1, +1>(st, w6, w6, 3, inv())));
     OpZRound<T, N, P3> orr{w6, CF, inv()};
-    LD_TRY((zrun<P3, 3, 3>(st, orr)));
-    LD_TRY((pass<1, -1>(st, w6, w6, 3, (T)1)));
+    const int nx = N / nslab;
+    for (int x0 = 0; x0 < N; x0 += nx) {
+      LD_TRY((pass<1, +1>(st, w6, w6, 3, inv(), x0, nx)));
+      LD_TRY((zrun<P3, 3, 3>(st, orr, -2, x0, nx)));
+      LD_TRY((pass<1, -1>(st, w6, w6, 3, (T)1, x0, nx)));
+    }
     hipLaunchKernelGGL((k_x_proj<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, (const C<T>*)w6, uk, CF,
                        (const C<T>*)tw, maxpart);
     LD_LAUNCHED();
@@ -560,8 +593,7 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL((k_x_rhs<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, ra, (const C<T>*)tw);
     LD_LAUNCHED();
     // the physical-space v the droplets see uses visc as well (forced-dns-sm.py:319-321)
-    LD_TRY(velocity_y(st));
-    LD_TRY(velocity_z(st, vphys, nullptr, (const T*)nin, clip_none));
+    LD_TRY(velocity(st, vphys, nullptr, (const T*)nin, clip_none, nullptr));
     LD_TRY(fill<T>(st, fcp, RF, (T)0));
     if (n_ > 0) {
       hipLaunchKernelGGL((k_prhs<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, coord, (int)n_,
@@ -569,12 +601,11 @@ struct Engine final : EngineBase {
                          wrow, dp);
       LD_LAUNCHED();
     }
-    LD_TRY(density_rhs(st, fcp));
     OpZKn<T, N, P1> ok;
     ok.w = wn; ok.fs = CF; ok.scale = inv(); ok.kn_out = (T*)kn;
     ok.n0 = nullptr; ok.nacc = nullptr; ok.nnext = nullptr; ok.wacc = (T)0; ok.anext = (T)0; ok.acc_init = 0;
     ok.sum_target = -1; ok.partials = partials2;
-    LD_TRY((zrun<P1, 1, 0>(st, ok)));
+    LD_TRY(density_rhs(st, fcp, &ok));
     return LD_OK;
   }
 
@@ -722,6 +753,44 @@ struct Engine final : EngineBase {
     LD_HIP(hipMemcpyAsync(&blown, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     LD_HIP(hipStreamSynchronize(st));
     if (blown) { set_error("Threshold exceeded: max Re(u_hat) > 100 N^3"); return LD_ERR_BLOWUP; }
+    return LD_OK;
+  }
+
+  // ------------------------------------------------------------ diagnostics
+  double* dpart = nullptr;
+  static constexpr int NBLK_DIAG = 256;
+  int diag_alloc() {
+    if (!dpart) LD_TRY(alloc(dpart, (size_t)NBLK_DIAG * (Nf > 4 ? Nf : 4)));
+    return LD_OK;
+  }
+  double norm0() const { return 1.0 / ((double)N * N * N * N * N * N / std::pow(2.0 * M_PI, 3.0)); }
+  int spectrum(const void* f, int nf, double w, double* out) override {
+    LD_TRY(diag_alloc());
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_shell_partial<T, N>), dim3(NBLK_DIAG), dim3(256), 0, st, (const C<T>*)f, (const C<T>*)nullptr,
+                       nf, CF, 0, w, norm0(), dpart);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL((k_shell_final<N>), dim3(1), dim3(256), 0, st, (const double*)dpart, NBLK_DIAG, 0, out);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  int flux(const void* ukin, const void* kuin, double* out) override {
+    LD_TRY(diag_alloc());
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_shell_partial<T, N>), dim3(NBLK_DIAG), dim3(256), 0, st, (const C<T>*)ukin,
+                       (const C<T>*)kuin, 3, CF, 1, 1.0, norm0(), dpart);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL((k_shell_final<N>), dim3(1), dim3(256), 0, st, (const double*)dpart, NBLK_DIAG, 1, out);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  int stats(const void* x, int nf, double* out) override {
+    LD_TRY(diag_alloc());
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_stats_partial<T>), dim3(NBLK_DIAG), dim3(256), 0, st, (const T*)x, (size_t)nf * RF, dpart);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, st, (const double*)dpart, NBLK_DIAG, out);
+    LD_LAUNCHED();
     return LD_OK;
   }
 

@@ -31,6 +31,9 @@ struct EngineBase {
                       int64_t max_pairs, int64_t* count) = 0;
   virtual int coalesce(double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
                        int64_t* nevents) = 0;
+  virtual int spectrum(const void* f, int nf, double w, double* out) = 0;
+  virtual int flux(const void* uk, const void* ku, double* out) = 0;
+  virtual int stats(const void* x, int nf, double* out) = 0;
   virtual int profile(int kernel, int enable) = 0;
   virtual int profile_read(double* total_ms, int64_t* launches) = 0;
   virtual int prhs(const double* coord, int64_t np, const void* u, const void* us, const void* n, void* fc,

@@ -97,11 +97,11 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
 // DIR > 0: inverse times `scale`.  In place is allowed (each block owns its lines).
 template <typename T, int N, int M, int AXIS, int DIR>
 __global__ void __launch_bounds__(M * Plan<N>::N1)
-k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale) {
+k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff) {
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, js = JS<N, AXIS>::v;
   __shared__ C<T> lds[M * (N + 1)];
   const int m = threadIdx.x % M, n1 = threadIdx.x / M;
-  const int q = blockIdx.x * M + m;
+  const int q = qoff + blockIdx.x * M + m;
   const C<T>* src = in + (size_t)blockIdx.y * fstride;
   C<T>* dst = out + (size_t)blockIdx.y * fstride;
   const uint32_t lb = (uint32_t)lineBase<N, AXIS>(q);
@@ -231,12 +231,12 @@ template <int N, int P, int FIN, int FOUT> struct ZThreads {
 // Op::storeSpec].  Op::loadSpec supplies the spectral inputs.
 template <typename T, int N, int P, int FIN, int FOUT, class Op>
 __global__ void __launch_bounds__((ZThreads<N, P, FIN, FOUT>::v))
-k_z(Op op, const C<T>* __restrict__ tw) {
+k_z(Op op, const C<T>* __restrict__ tw, int lineoff) {
   constexpr int Nf = N / 2 + 1;
   constexpr int NREG = ZMax<FIN, FOUT>::v * P / 2;
   __shared__ C<T> lds[NREG * 2 * Nf];
   ZTile<T, N, P> tile{lds};
-  const int line0 = blockIdx.x * P;
+  const int line0 = lineoff + blockIdx.x * P;
   const int nt = blockDim.x;
   if constexpr (FIN > 0) {
     for (int e = threadIdx.x; e < FIN * P * Nf; e += nt) {
@@ -402,11 +402,12 @@ struct OpZKn {
     if (sum_target >= 0) {
       constexpr int NT = ZThreads<N, P, 1, 0>::v;
       __shared__ double sh[NT];
+      const int blk = line0 / P;            // global tile index (launches may cover a slab)
       const double a = blockSum<NT>(s_old, sh);
-      if (threadIdx.x == 0) partials[2 * blockIdx.x] = a;
+      if (threadIdx.x == 0) partials[2 * blk] = a;
       __syncthreads();
       const double b = blockSum<NT>(s_new, sh);
-      if (threadIdx.x == 0) partials[2 * blockIdx.x + 1] = b;
+      if (threadIdx.x == 0) partials[2 * blk + 1] = b;
     }
   }
   __device__ void storeSpec(int, int, int, C<T>) const {}
@@ -424,33 +425,43 @@ k_x_uw(const C<T>* __restrict__ uk, C<T>* __restrict__ out, size_t fs, const C<T
   const int q = blockIdx.x * M + m;
   const T ky = (T)kwave(q / Nf, N), kz = (T)(q % Nf);
   C<T>* L = lds + m * (N + 1);
-  const C<T>* U0 = uk; const C<T>* U1 = uk + fs; const C<T>* U2 = uk + 2 * fs;
-  C<T> u0[N2], u1[N2], u2[N2];
+  C<T> u0[N2], u1[N2], u2[N2], a[N2];
 #pragma unroll
   for (int r = 0; r < N2; ++r) {
     const uint32_t o = (uint32_t)(q + posOut<N>(n1, r) * js);
-    u0[r] = ld32(U0, o); u1[r] = ld32(U1, o); u2[r] = ld32(U2, o);
+    u0[r] = ld32(uk, o); u1[r] = ld32(uk + fs, o); u2[r] = ld32(uk + 2 * fs, o);
   }
-#pragma unroll 1
-  for (int comp = 0; comp < 6; ++comp) {
-    C<T> a[N2];
-#pragma unroll
-    for (int r = 0; r < N2; ++r) {
-      const T kx = (T)kwave(posOut<N>(n1, r), N);
-      C<T> s;
-      if (comp == 0) s = u0[r];
-      else if (comp == 1) s = u1[r];
-      else if (comp == 2) s = u2[r];
-      else if (comp == 3) s = cmuli<T>(mkc<T>(ky * u2[r].x - kz * u1[r].x, ky * u2[r].y - kz * u1[r].y));
-      else if (comp == 4) s = cmuli<T>(mkc<T>(kz * u0[r].x - kx * u2[r].x, kz * u0[r].y - kx * u2[r].y));
-      else s = cmuli<T>(mkc<T>(kx * u1[r].x - ky * u0[r].x, kx * u1[r].y - ky * u0[r].y));
-      a[r] = s;
-    }
-    lineInv<T, N>(a, L, n1, true, tw);
+  auto store = [&](int comp, const C<T>* v) {
     C<T>* O = out + comp * fs;
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) st32(O, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
+    for (int n2 = 0; n2 < N2; ++n2) st32(O, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(v[n2], scale));
+  };
+  // vorticity i k x u first (all three components of u are live), then u in place
+#pragma unroll
+  for (int r = 0; r < N2; ++r)
+    a[r] = cmuli<T>(mkc<T>(ky * u2[r].x - kz * u1[r].x, ky * u2[r].y - kz * u1[r].y));
+  lineInv<T, N>(a, L, n1, true, tw);
+  store(3, a);
+#pragma unroll
+  for (int r = 0; r < N2; ++r) {
+    const T kx = (T)kwave(posOut<N>(n1, r), N);
+    a[r] = cmuli<T>(mkc<T>(kz * u0[r].x - kx * u2[r].x, kz * u0[r].y - kx * u2[r].y));
   }
+  lineInv<T, N>(a, L, n1, true, tw);
+  store(4, a);
+#pragma unroll
+  for (int r = 0; r < N2; ++r) {
+    const T kx = (T)kwave(posOut<N>(n1, r), N);
+    a[r] = cmuli<T>(mkc<T>(kx * u1[r].x - ky * u0[r].x, kx * u1[r].y - ky * u0[r].y));
+  }
+  lineInv<T, N>(a, L, n1, true, tw);
+  store(5, a);
+  lineInv<T, N>(u0, L, n1, true, tw);
+  store(0, u0);
+  lineInv<T, N>(u1, L, n1, true, tw);
+  store(1, u1);
+  lineInv<T, N>(u2, L, n1, true, tw);
+  store(2, u2);
 }
 
 // Arguments of the fused spectral RHS pass.

@@ -226,6 +226,35 @@ class ForcedDNS:
         self._stream()
         _lib.check(self.lib.ld_advance(self.ctx, float(h), int(nsteps)))
 
+    # ------------------------------------------------------------ diagnostics (save(), plot.py)
+    def spectrum(self, fk: torch.Tensor, weight: float = 0.5) -> torch.Tensor:
+        """e3d_to_e1d(weight * sum_c |f_c|^2 * normalize): E(k) for u_hat with weight 0.5
+        (forced-dns-sm.py:423, 431), the n spectrum with weight 1 (plot.py:37-38)."""
+        nf = 1 if fk.dim() == 3 else fk.shape[0]
+        self._chk(fk, self.cdtype, self.cshape(None if fk.dim() == 3 else nf), "fk")
+        out = torch.zeros(self.Nf, dtype=torch.float64, device=self.device)
+        self._stream()
+        _lib.check(self.lib.ld_spectrum(self.ctx, _ptr(fk), nf, float(weight), _ptr(out)))
+        return out
+
+    def flux(self, uk: torch.Tensor, n: torch.Tensor) -> torch.Tensor:
+        """Pi(k) of save() (forced-dns-sm.py:425-428): full_RHS with visc = forc = 0
+        (no droplets), Re(conj(uk) . ku) dealiased, shell-binned, cumulated from high k."""
+        ku, _, _, _ = self.full_RHS(0.0, uk, n, None, None, visc=0, forc=0)
+        out = torch.zeros(self.Nf, dtype=torch.float64, device=self.device)
+        self._stream()
+        _lib.check(self.lib.ld_flux(self.ctx, _ptr(uk), _ptr(ku), _ptr(out)))
+        return out
+
+    def field_stats(self, x: torch.Tensor):
+        """(sum, min, max, sum of squares) of one or several real fields."""
+        nf = 1 if x.dim() == 3 else x.shape[0]
+        self._chk(x, self.rdtype, self.rshape(None if x.dim() == 3 else nf), "x")
+        out = torch.zeros(4, dtype=torch.float64, device=self.device)
+        self._stream()
+        _lib.check(self.lib.ld_field_stats(self.ctx, _ptr(x), nf, _ptr(out)))
+        return tuple(out.cpu().tolist())
+
     def advance_plain(self, h, nsteps: int = 1):
         """nsteps with the reference's unfused dataflow (ld_advance_plain)."""
         self._stream()

@@ -273,3 +273,34 @@ def test_collision_capacity_error(gpu):
     with pytest.raises(_lib.LdError):
         stb.collision_pairs(torch.full((500,), 0.5, dtype=torch.float64, device=gpu), max_pairs=4)
     d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_diagnostics(gpu, prec):
+    """E(k), the n spectrum, Pi(k) and field statistics of save() / plot.py."""
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(32, prec)
+    uk, n, coord, pid = inputs(cfg, seed=21, n_noise=0.6)
+    fl = make_flow_oracle(cfg)
+    G = fl.grid
+    d = ForcedDNS(cfg, max_particles=0)
+    ukg = _t(uk, gpu, d.cdtype)
+    ng = _t(n, gpu, d.rdtype)
+    tol = 1e-12 if prec == "f64" else 2e-5
+    e = d.spectrum(ukg, 0.5).cpu().numpy()
+    assert relerr(e, G.e3d_to_e1d(G.energy3d(uk))) < tol
+    nk = G.rfft_mpi(n)
+    nkg = d.zeros_spec()
+    d.rfft_mpi(ng, nkg)
+    assert relerr(d.spectrum(nkg, 1.0).cpu().numpy(), G.e3d_to_e1d(np.abs(nk) ** 2 * G.normalize)) < 10 * tol
+    ku, _, _, _ = fl.full_RHS(0.0, uk, n.copy(), np.zeros((0, 7)), np.zeros((0, 7)), None, visc=0, forc=0)
+    pik = np.real(np.conj(uk[0]) * ku[0] + np.conj(uk[1]) * ku[1] + np.conj(uk[2]) * ku[2]) * G.dealias * G.normalize
+    ref = np.cumsum(G.e3d_to_e1d(pik)[::-1])[::-1]
+    got = d.flux(ukg, ng).cpu().numpy()
+    assert np.max(np.abs(got - ref)) < (1e-9 if prec == "f64" else 1e-3) * np.max(np.abs(ref))
+    s, mn, mx, sq = d.field_stats(ng)
+    nn = ng.double().cpu().numpy()
+    assert abs(s - nn.sum()) < 1e-9 * abs(nn.sum()) and mn == nn.min() and mx == nn.max()
+    assert abs(sq - (nn ** 2).sum()) < 1e-9 * (nn ** 2).sum()
+    d.close()
