@@ -304,3 +304,43 @@ def test_diagnostics(gpu, prec):
     assert abs(s - nn.sum()) < 1e-9 * abs(nn.sum()) and mn == nn.min() and mx == nn.max()
     assert abs(sq - (nn ** 2).sum()) < 1e-9 * (nn ** 2).sum()
     d.close()
+
+
+def test_c1_200_steps_against_oracle_golden(gpu):
+    """The 64^3 FP64 workload (10^4 droplets, 200 steps) against the committed
+    CPU-oracle run (tests/golden/make_c1_golden.py): 1e-10 relative on the
+    sampled spectral modes, density points and droplet rows (by id)."""
+    import os
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    from lucky_droplets_amd.dns import ForcedDNS
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c1_oracle_200.npz"))
+    cfg = preset("c1")
+    rng = np.random.default_rng(cfg.seed)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    assert dt == float(g["dt"])
+    coord, pid = ic.initial_droplets(cfg, rng)
+    d = ForcedDNS(cfg)
+    ukg = _t(uk, gpu, d.cdtype)
+    ng = _t(n, gpu, d.rdtype)
+    cg = _t(coord, gpu, torch.float64)
+    d.set_state(ukg, ng, cg)
+    d.advance(dt, int(g["steps"]))
+    d.get_state(ukg, ng, cg)
+    torch.cuda.synchronize()
+    uk1, n1, c1 = ukg.cpu().numpy(), ng.cpu().numpy(), cg.cpu().numpy()
+    mi, ni = g["mode_ix"], g["n_ix"]
+    e_u = np.max(np.abs(uk1[:, mi[0], mi[1], mi[2]] - g["uk_modes"])) / float(g["uk_absmax"])
+    e_n = np.max(np.abs(n1[ni[0], ni[1], ni[2]] - g["n_pts"])) / float(g["n_absmax"])
+    rows = c1[g["droplet_ids"]]            # rows come back in input order = id order
+    ref = g["droplet_rows"]
+    e_x = min_image(rows, ref, cfg.L) / cfg.L
+    e_v = relerr(rows[:, 3:6], ref[:, 3:6])
+    e_m = relerr(rows[:, 6], ref[:, 6])
+    print(f"c1 200 steps: uk {e_u:.2e} n {e_n:.2e} x {e_x:.2e} v {e_v:.2e} m {e_m:.2e}")
+    assert max(e_u, e_n, e_x, e_v, e_m) < 1e-10
+    ek = d.spectrum(ukg, 0.5).cpu().numpy()
+    assert relerr(ek, g["ek"]) < 1e-10
+    d.close()
