@@ -71,6 +71,7 @@ struct Engine final : EngineBase {
   std::vector<void*> allocs;
   C<T>* tw = nullptr;
   hipStream_t st2 = nullptr;        // density/droplet stream (created here)
+  bool serial = false;              // no overlap of the flow and density chains
   hipEvent_t evA = nullptr, evB = nullptr;
   C<T>*uk = nullptr, *us = nullptr, *acc = nullptr, *w6 = nullptr, *wn = nullptr;
   T* ubuf[2] = {nullptr, nullptr};       // planar physical u (API / plain paths)
@@ -136,6 +137,9 @@ struct Engine final : EngineBase {
     }
     if (p.max_particles < 0 || p.max_particles > (int64_t)0x7fffffff) { set_error("max_particles out of range"); return LD_ERR_ARG; }
     cap = p.max_particles;
+    // LD_SERIAL=1: the flow stream waits for each stage's density chain, so the
+    // two chains never run concurrently (per-kernel timing without overlap)
+    if (const char* e = getenv("LD_SERIAL")) serial = atoi(e) != 0;
     if (const char* e = getenv("LD_SLABS")) {
       const int v = atoi(e);
       if (v >= 1 && N % v == 0 && (N / v) % M == 0) nslab = v;
@@ -417,7 +421,7 @@ struct Engine final : EngineBase {
       LD_LAUNCHED();
       LD_TRY(prof_mark(LD_K_DROPLETS, st2));
       if (sorted_ok) {
-        hipLaunchKernelGGL((k_brick_deposit<T, N>), dim3(Brick<N>::NB * Brick<N>::NB * Brick<N>::NB), dim3(256), 0,
+        hipLaunchKernelGGL((k_brick_deposit<T, N>), dim3(Brick<N>::NB * Brick<N>::NB * Brick<N>::NB), dim3(Brick<N>::NT), 0,
                            st2, (const double4*)dep, (const int*)hist, (int)np, fc, dp);
         LD_LAUNCHED();
       }
@@ -436,6 +440,7 @@ struct Engine final : EngineBase {
                        1.0 / ((double)N * N * N), (s < 3) ? clip_stage : clip_end);
     LD_LAUNCHED();
     LD_HIP(hipEventRecord(evB, st2));
+    if (serial) LD_TRY(join(st, st2, evB));
     return LD_OK;
   }
 
@@ -443,6 +448,7 @@ struct Engine final : EngineBase {
   int begin_rk4() {
     LD_TRY(clip_sums(st2, n_state, clip_stage));
     LD_HIP(hipEventRecord(evB, st2));
+    if (serial) LD_TRY(join(st, st2, evB));
     return LD_OK;
   }
 

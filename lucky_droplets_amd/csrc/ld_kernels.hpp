@@ -101,7 +101,11 @@ k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, js = JS<N, AXIS>::v;
   __shared__ C<T> lds[M * (N + 1)];
   const int m = threadIdx.x % M, n1 = threadIdx.x / M;
-  const int q = qoff + blockIdx.x * M + m;
+  // XCD-aware: consecutive line groups go to one XCD, so row segments that
+  // straddle cache lines (y rows start every Nf elements) meet in its L2
+  const int nbx = gridDim.x;
+  const int bx = (nbx % 8 == 0) ? (blockIdx.x % 8) * (nbx / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int q = qoff + bx * M + m;
   const C<T>* src = in + (size_t)blockIdx.y * fstride;
   C<T>* dst = out + (size_t)blockIdx.y * fstride;
   const uint32_t lb = (uint32_t)lineBase<N, AXIS>(q);

@@ -14,21 +14,29 @@ namespace ld {
 // w = (1 + cos((x - X_j) pi / (2 dx))) / 4 (particles.py:12, 16, 456).  X_j is
 // taken unwrapped (j*dx), which equals the reference's wrapped X[j mod N]
 // through the period 4*dx of the kernel (N % 4 == 0).
-__device__ __forceinline__ double cosW(double x, int j, double dx, double pi_2dx) {
-  return 0.25 * (1.0 + cos((x - (double)j * dx) * pi_2dx));
-}
 template <int N> __device__ __forceinline__ int wrapIdx(int j) {
   int jj = j % N;
   return jj < 0 ? jj + N : jj;
 }
+// The four weights of one axis from one sincos: with
+// theta = (x - i0 dx) pi / (2 dx), i0 = floor(x/dx) - 1, node i0 + a has the
+// argument theta - a pi/2, so its weight is (1 + {cos, sin, -cos, -sin}(theta))/4.
+// The four weights then sum to 1 in exact arithmetic.  Returns i0.
+__device__ __forceinline__ int cosW4(double x, double dx, double pi_2dx, double w[4]) {
+  const int i0 = (int)floor(x / dx) - 1;
+  double sn, cs;
+  sincos((x - (double)i0 * dx) * pi_2dx, &sn, &cs);
+  w[0] = 0.25 * (1.0 + cs);
+  w[1] = 0.25 * (1.0 + sn);
+  w[2] = 0.25 * (1.0 - cs);
+  w[3] = 0.25 * (1.0 - sn);
+  return i0;
+}
 template <int N>
 __device__ __forceinline__ void cosWeights(double x, double dx, double pi_2dx, int idx[4], double w[4]) {
-  const int i = (int)floor(x / dx);
+  const int i0 = cosW4(x, dx, pi_2dx, w);
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    w[a] = cosW(x, i - 1 + a, dx, pi_2dx);
-    idx[a] = wrapIdx<N>(i - 1 + a);
-  }
+  for (int a = 0; a < 4; ++a) idx[a] = wrapIdx<N>(i0 + a);
 }
 
 // Periodic wrap of one coordinate, x mod L with numpy's sign convention
@@ -41,21 +49,18 @@ __device__ __forceinline__ double wrapL(double x, double L) {
   return r;
 }
 
-// Geometry of one droplet's stencil.  The x weights are evaluated inside the
-// x loop (xa(), wxa()) so the loop need not be unrolled.
+// Geometry of one droplet's stencil.
 template <int N> struct Stencil {
-  double x, dx, pi_2dx;
   int i0;
   int iy[4], iz[4];
-  double wy[4], wz[4];
-  __device__ __forceinline__ void set(const double xw[3], double dx_, double pi_2dx_) {
-    x = xw[0]; dx = dx_; pi_2dx = pi_2dx_;
-    i0 = (int)floor(x / dx) - 1;
+  double wx[4], wy[4], wz[4];
+  __device__ __forceinline__ void set(const double xw[3], double dx, double pi_2dx) {
+    i0 = cosW4(xw[0], dx, pi_2dx, wx);
     cosWeights<N>(xw[1], dx, pi_2dx, iy, wy);
     cosWeights<N>(xw[2], dx, pi_2dx, iz, wz);
   }
   __device__ __forceinline__ int xa(int a) const { return wrapIdx<N>(i0 + a); }
-  __device__ __forceinline__ double wxa(int a) const { return cosW(x, i0 + a, dx, pi_2dx); }
+  __device__ __forceinline__ double wxa(int a) const { return wx[a]; }
 };
 
 // Gather of NC fields f[c*R + idx] (particles.py:6-20, 454-509): for each x
@@ -159,11 +164,23 @@ __device__ __forceinline__ void dropletRhs(const DropPhys& P, const double vp[3]
 // ------------------------------------------------------------ spatial sort
 // Droplets are reordered by spatial bin once per step (counting sort), so
 // that lanes of a wavefront gather from and deposit into the same cache
-// lines.  Bins are cubes of BS = max(2, N/128) grid cells.
+// lines.  Bins are cubes of BS = max(2, N/128) grid cells; bins are numbered
+// tile-major (tiles of TS^3 bins = TC^3 cells, the deposit bricks), so the
+// droplets of one tile are contiguous and a run of gather blocks covers a
+// compact region of the fields.
 template <int N> struct Bins {
   static constexpr int BS = (N / 128 > 2) ? N / 128 : 2;
   static constexpr int B = N / BS;
   static constexpr int NB = B * B * B;
+  static constexpr int TC = (N >= 128) ? 16 : 8;    // tile side in cells
+  static constexpr int TS = TC / BS;                 // tile side in bins
+  static constexpr int NTt = B / TS;                 // tiles per side
+  static_assert(TC % BS == 0 && B % TS == 0, "bin tiles");
+  // key of bin (bx, by, bz)
+  static __device__ __forceinline__ int key(int bx, int by, int bz) {
+    const int tile = ((bx / TS) * NTt + by / TS) * NTt + bz / TS;
+    return tile * (TS * TS * TS) + ((bx % TS) * TS + by % TS) * TS + bz % TS;
+  }
 };
 
 // ------------------------------------------------------------ fused RK stage
@@ -218,12 +235,17 @@ k_particles_stage(PStage<T, PT> A) {
   if (A.stage > 0) m = (double)(PT)(A.v0[3 * cap + pp] + (PT)A.ah * A.kv[3 * cap + pp]);
   const double L = A.P.L, dx = A.P.dx, pi_2dx = A.P.pi_2dx;
   const double xw = wrapL(xs[0], L), yw = wrapL(xs[1], L), zw = wrapL(xs[2], L);
-  const int i0 = (int)floor(xw / dx) - 1, j0 = (int)floor(yw / dx) - 1, k0 = (int)floor(zw / dx) - 1;
+  // one sincos per axis (cosW4); node i0 + a has weight (1 +- {cos, sin})/4
+  const int i0 = (int)floor(xw / dx) - 1, k0 = (int)floor(zw / dx) - 1;
+  double xsn, xcs, zsn, zcs;
+  sincos((xw - (double)i0 * dx) * pi_2dx, &xsn, &xcs);
+  sincos((zw - (double)k0 * dx) * pi_2dx, &zsn, &zcs);
   double wy[4];
   int iy[4];
+  const int j0 = cosW4(yw, dx, pi_2dx, wy);
 #pragma unroll
-  for (int b = 0; b < 4; ++b) { wy[b] = cosW(yw, j0 + b, dx, pi_2dx); iy[b] = wrapIdx<N>(j0 + b); }
-  const double wz = cosW(zw, k0 + c, dx, pi_2dx);
+  for (int b = 0; b < 4; ++b) iy[b] = wrapIdx<N>(j0 + b);
+  const double wz = 0.25 * (1.0 + ((c & 2) ? -1.0 : 1.0) * ((c & 1) ? zsn : zcs));
   const int iz = wrapIdx<N>(k0 + c);
   // gather: this lane's z node, all (x, y) nodes; two 16-byte records per node
   double I[7];
@@ -232,7 +254,7 @@ k_particles_stage(PStage<T, PT> A) {
   if (live) {
 #pragma unroll 1
     for (int a = 0; a < 4; ++a) {
-      const double wx = cosW(xw, i0 + a, dx, pi_2dx);
+      const double wx = 0.25 * (1.0 + ((a & 2) ? -1.0 : 1.0) * ((a & 1) ? xsn : xcs));
       const size_t plane = (size_t)wrapIdx<N>(i0 + a) * N;
       double t[7];
 #pragma unroll
@@ -307,77 +329,200 @@ k_particles_stage(PStage<T, PT> A) {
 
 // ------------------------------------------------------------ brick deposit
 // Deposit of W with the cosine kernel / dx^3 (particles.py:23-35, 529-575),
-// organised by output: one workgroup per brick of 8^3 grid nodes scans the
-// sorted-bin ranges that can reach the brick (stencil -1..+2 cells, plus one
-// bin of motion since the sort), accumulates in LDS and writes each node of
-// the brick exactly once.  Slots on the fallback list are skipped here.
+// organised by output and free of atomics: one workgroup per brick of S^3
+// grid nodes; thread (y, z) owns the node column x = 0..S-1 of the brick in
+// registers and writes it once.
+//
+// The droplets that can reach the brick are those whose stage cell lies in
+// [c0 - 2, c0 + S] per axis; their sorted bins (bins of the position at the
+// last sort) lie in that cell range's bins widened by one bin of motion
+// (slots that moved further are on the fallback list and skipped here).  For
+// each (bin x, bin y) of that window the z-run of bins is one contiguous range
+// of sorted slots per bin tile it crosses, so the candidates form at most
+// NZS * NBX^2 ranges.  Per round, CH candidates are tested for overlap with the
+// brick; those that overlap are compacted into an LDS list with their stencil
+// origin and weights, and every wavefront then walks the list, skipping
+// droplets that miss its rows.
 template <int N> struct Brick {
-  static constexpr int S = 8;                       // nodes per side
-  static constexpr int NB = N / S;                  // bricks per side
+  static constexpr int S = Bins<N>::TC;              // nodes per side (one bin tile)
+  static constexpr int NB = N / S;                   // bricks per side
+  static constexpr int NT = S * S;                   // threads: one per (y, z) column
+  // bins spanned by S + 3 stage cells from a bin-aligned start, +-1 bin of motion
+  static constexpr int W0 = (S + 2 + Bins<N>::BS - 1) / Bins<N>::BS + 1 + 2;
+  static constexpr int NBX = W0 < Bins<N>::B ? W0 : Bins<N>::B;
+  // z-runs of NBX bins split at tile boundaries into at most NZS key ranges
+  static constexpr int NZS = (NBX + Bins<N>::TS - 1) / Bins<N>::TS + 1;
+  static constexpr int NR = NZS * NBX * NBX;         // slot ranges
+  static_assert(N % S == 0 && (NB * NB * NB) % 8 == 0, "brick grid");
 };
 
+// Packed brick-relative stencil origin (each in [-3, S-1]).
+__device__ __forceinline__ int packOrg(int rx, int ry, int rz) { return (rx + 3) | ((ry + 3) << 8) | ((rz + 3) << 16); }
+
 template <typename T, int N>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(Brick<N>::NT)
 k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, int np, T* __restrict__ fc,
                 DropPhys P) {
-  constexpr int S = Brick<N>::S, NBr = Brick<N>::NB;
+  using Bk = Brick<N>;
+  constexpr int S = Bk::S, NBr = Bk::NB, NBX = Bk::NBX, NR = Bk::NR, NT = Bk::NT, NZS = Bk::NZS;
+  constexpr int TS = Bins<N>::TS;
+  constexpr int NW = NT / 64;                               // wavefronts
+  constexpr int RPW = 64 / S;                               // brick rows (y) per wavefront
+  constexpr int CH = (sizeof(T) == 4 ? 4 : 2) * NT;         // candidates per round
   constexpr int BS = Bins<N>::BS, B = Bins<N>::B;
-  __shared__ float tile_f[(sizeof(T) == 4) ? S * S * S : 1];
-  __shared__ double tile_d[(sizeof(T) == 8) ? S * S * S : 1];
-  T* tile = (sizeof(T) == 4) ? (T*)tile_f : (T*)tile_d;
-  const int br = blockIdx.x;
+  constexpr int NBIN = B * B * B;
+  __shared__ int rb[NR], rn[NR], rl[NR + 1];
+  __shared__ int lorg[CH];
+  __shared__ V4<T> lwx[CH];      // W inv_vol * w_x(a), a = 0..3
+  __shared__ V4<T> lyz[CH];      // sin, cos of the y and z stencil angles
+  __shared__ int cnt;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // XCD-aware: each of the 8 XCDs takes a contiguous run of bricks
+  const int nb = gridDim.x, per = nb / 8;
+  const int br = (blockIdx.x % 8) * per + blockIdx.x / 8;
   const int c0x = (br / (NBr * NBr)) * S, c0y = ((br / NBr) % NBr) * S, c0z = (br % NBr) * S;
-  for (int i = threadIdx.x; i < S * S * S; i += blockDim.x) tile[i] = (T)0;
-  __syncthreads();
-  // bins whose droplets can reach the brick: stage cells [c0-2, c0+S] -> bins, +-1 bin of motion
-  const int blx = (c0x - 2 + N) % N / BS, bly = (c0y - 2 + N) % N / BS, blz = (c0z - 2 + N) % N / BS;
-  // stage cells c0-2 .. c0+S span at most (S + 2 + BS - 1)/BS + 1 bins; one more on each side for motion
-  int nbx = (S + 2 + BS - 1) / BS + 1 + 2;
-  if (nbx > B) nbx = B;
-  const int lo_shift = (nbx == B) ? 0 : 1;          // start one bin below the stencil's lowest bin
-  const int nruns = nbx * nbx;
-  const double dx = P.dx, pi_2dx = P.pi_2dx;
-  for (int r = threadIdx.x >> 2; r < nruns; r += blockDim.x >> 2) {
-    const int bx = (blx - lo_shift + r / nbx + B) % B, by = (bly - lo_shift + r % nbx + B) % B;
-    for (int q = 0; q < nbx; ++q) {
-      const int bz = (blz - lo_shift + q + B) % B;
-      const int bin = (bx * B + by) * B + bz;
-      const int b = start[bin], e = (bin + 1 < B * B * B) ? start[bin + 1] : np;
-      for (int t = b + (threadIdx.x & 3); t < e; t += 4) {
-        const double4 d = dep[t];
-        if (!(d.w > 0.0)) continue;             // no growth, or a fallback slot
-        const int i0 = (int)floor(d.x / dx) - 1, j0 = (int)floor(d.y / dx) - 1, k0 = (int)floor(d.z / dx) - 1;
-        // brick-relative first node of the stencil (periodic); reject without overlap
-        const int rx = ((i0 - c0x) % N + N) % N, ry = ((j0 - c0y) % N + N) % N, rz = ((k0 - c0z) % N + N) % N;
-        const bool ox = rx < S || rx > N - 4, oy = ry < S || ry > N - 4, oz = rz < S || rz > N - 4;
-        if (!(ox && oy && oz)) continue;
-        const double W = d.w * P.inv_vol;
+  if (tid == 0) cnt = 0;
+  // slot ranges of the window's z-runs
+  const int lo = (NBX == B) ? 0 : 1;
+  const int blx = ((c0x - 2 + N) % N) / BS - lo, bly = ((c0y - 2 + N) % N) / BS - lo;
+  const int z0 = (NBX == B) ? 0 : (((c0z - 2 + N) % N) / BS - lo + B) % B;
+  for (int r = tid; r < NBX * NBX; r += NT) {
+    const int bx = (blx + r / NBX + B) % B, by = (bly + r % NBX + B) % B;
+    int bz = z0, rem = NBX;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int nx = (rx + a) % N;
-          if (nx >= S) continue;
-          const double wa = W * cosW(d.x, i0 + a, dx, pi_2dx);
-#pragma unroll
-          for (int bb = 0; bb < 4; ++bb) {
-            const int ny = (ry + bb) % N;
-            if (ny >= S) continue;
-            const double wab = wa * cosW(d.y, j0 + bb, dx, pi_2dx);
-#pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-              const int nz = (rz + cc) % N;
-              if (nz >= S) continue;
-              atomicAdd(&tile[(nx * S + ny) * S + nz], (T)(wab * cosW(d.z, k0 + cc, dx, pi_2dx)));
-            }
-          }
-        }
+    for (int sgm = 0; sgm < NZS; ++sgm) {
+      int b0 = 0, len = 0;
+      if (rem > 0) {
+        const int l = bz % TS;
+        const int nz = (TS - l < rem) ? TS - l : rem;   // bins of this run inside one tile
+        const int k0 = Bins<N>::key(bx, by, bz);        // keys k0 .. k0 + nz - 1 are consecutive
+        b0 = start[k0];
+        len = ((k0 + nz < NBIN) ? start[k0 + nz] : np) - b0;
+        bz = (bz + nz) % B;
+        rem -= nz;
       }
+      rb[NZS * r + sgm] = b0; rn[NZS * r + sgm] = len;
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < S * S * S; i += blockDim.x) {
-    const int nx = i / (S * S), ny = (i / S) % S, nz = i % S;
-    fc[((size_t)(c0x + nx) * N + (c0y + ny)) * N + (c0z + nz)] = tile[i];
+  // exclusive prefix of the range lengths (wavefront 0)
+  if (tid < 64) {
+    constexpr int PER = (NR + 63) / 64;
+    int loc[PER];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = tid * PER + k;
+      loc[k] = sum;
+      sum += (idx < NR) ? rn[idx] : 0;
+    }
+    int inc = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(inc, off, 64);
+      if (tid >= off) inc += t;
+    }
+    const int ex = inc - sum;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = tid * PER + k;
+      if (idx < NR) rl[idx] = ex + loc[k];
+    }
+    if (tid == 63) rl[NR] = inc;
   }
+  __syncthreads();
+  const int total = rl[NR];
+  const double dx = P.dx, pi_2dx = P.pi_2dx;
+  const int y = tid / S, z = tid % S;
+  const int yw0 = wv * RPW;                                 // first row of this wavefront
+  T acc[S];
+#pragma unroll
+  for (int x = 0; x < S; ++x) acc[x] = (T)0;
+  for (int base = 0; base < total; base += CH) {
+    // test and compact: overlapping, growing candidates enter the list
+#pragma unroll 1
+    for (int k = 0; k < CH / NT; ++k) {
+      const int f = base + k * NT + tid;
+      bool take = false;
+      int org = 0;
+      double4 d = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (f < total) {
+        int a = 0, b = NR;                                  // rl[a] <= f < rl[b]
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (rl[mid] <= f) a = mid; else b = mid;
+        }
+        d = dep[rb[a] + (f - rl[a])];
+        if (d.w > 0.0) {
+          int rr[3];
+          const double xb[3] = {d.x, d.y, d.z};
+          const int cb[3] = {c0x, c0y, c0z};
+          take = true;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            int r = (((int)floor(xb[c] / dx) - 1 - cb[c]) % N + N) % N;
+            if (r > N - 4) r -= N;                          // origin below the brick: -3..-1
+            take = take && r < S;
+            rr[c] = r;
+          }
+          org = packOrg(rr[0], rr[1], rr[2]);
+        }
+      }
+      const unsigned long long m = __ballot(take);
+      const int nw = __popcll(m);
+      int wbase = 0;
+      if (lane == 0 && nw) wbase = atomicAdd(&cnt, nw);
+      wbase = __shfl(wbase, 0, 64);
+      if (take) {
+        const int slot = wbase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        double sx, cx, sy, cy, sz, cz;
+        const int i0 = (int)floor(d.x / dx) - 1, j0 = (int)floor(d.y / dx) - 1, k0 = (int)floor(d.z / dx) - 1;
+        sincos((d.x - (double)i0 * dx) * pi_2dx, &sx, &cx);
+        sincos((d.y - (double)j0 * dx) * pi_2dx, &sy, &cy);
+        sincos((d.z - (double)k0 * dx) * pi_2dx, &sz, &cz);
+        const double W = d.w * P.inv_vol * (1.0 / 64.0);  // the three factors 1/4 of the weights
+        lorg[slot] = org;
+        lwx[slot] = V4<T>{(T)(W * (1.0 + cx)), (T)(W * (1.0 + sx)), (T)(W * (1.0 - cx)), (T)(W * (1.0 - sx))};
+        lyz[slot] = V4<T>{(T)sy, (T)cy, (T)sz, (T)cz};
+      }
+    }
+    __syncthreads();
+    const int n = cnt;
+    for (int i = 0; i < n; ++i) {
+      const int org = __builtin_amdgcn_readfirstlane(lorg[i]);
+      const int ry = ((org >> 8) & 255) - 3;
+      if (ry + 3 < yw0 || ry >= yw0 + RPW) continue;        // misses this wavefront's rows
+      const int rx = (org & 255) - 3, rz = ((org >> 16) & 255) - 3;
+      const int dy = y - ry, dz = z - rz;
+      const V4<T> q = lyz[i];
+      // 4 w_y(dy) * 4 w_z(dz) with 4 w(a) = 1 + {cos, sin, -cos, -sin}; the
+      // x weights carry W / dx^3 and the three factors 1/4
+      T ty = (dy & 1) ? q.x : q.y;
+      ty = (dy & 2) ? -ty : ty;
+      T tz = (dz & 1) ? q.z : q.w;
+      tz = (dz & 2) ? -tz : tz;
+      const bool hit = (unsigned)dy < 4u && (unsigned)dz < 4u;
+      const T v = hit ? ((T)1 + ty) * ((T)1 + tz) : (T)0;
+      const V4<T> wx = lwx[i];
+      const T wxa[4] = {wx.x, wx.y, wx.z, wx.w};
+      // rx is uniform: one scalar branch picks the (at most 4) columns
+      SFor<-3, S>::run([&](auto xc) {
+        constexpr int X = decltype(xc)::value;
+        if (rx == X) {
+          SFor<0, 4>::run([&](auto ac) {
+            constexpr int A = decltype(ac)::value;
+            if constexpr (X + A >= 0 && X + A < S) acc[X + A] += v * wxa[A];
+          });
+        }
+      });
+    }
+    __syncthreads();
+    if (tid == 0) cnt = 0;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < S; ++x) fc[((size_t)(c0x + x) * N + (c0y + y)) * N + (c0z + z)] = acc[x];
 }
 
 // Fallback deposit (global atomics) of the slots that moved more than one
@@ -494,7 +639,7 @@ k_bin_count(const double* __restrict__ pos, int np, int cap, double L, double dx
     ci = ci < 0 ? 0 : (ci > N - 1 ? N - 1 : ci);
     b[c] = ci / Bins<N>::BS;
   }
-  const int k = (b[0] * Bins<N>::B + b[1]) * Bins<N>::B + b[2];
+  const int k = Bins<N>::key(b[0], b[1], b[2]);
   key[p] = k;
   rank[p] = atomicAdd(&hist[k], 1);
 }
