@@ -172,6 +172,16 @@ int ld_field_stats(ld_ctx* ctx, const void* x, int32_t nfields, double* out);
  * if it exceeds max_pairs. */
 int ld_collide_pairs(ld_ctx* ctx, const double* rows, const double* radius, int64_t np, double rmax,
                      int64_t* pairs, int64_t max_pairs, int64_t* npairs);
+/* Pair statistics within the cutoff rmax (0 < rmax <= L/2), the input of the
+ * radial distribution function and the kinematic collision kernel; no
+ * counterpart in the reference.  For every pair i < j with minimum-image
+ * d2 < rmax^2: dist = sqrt(d2), bin k = min((int)(dist / (rmax / nbins)),
+ * nbins - 1), w_r = ((dvx*dx + dvy*dy) + dvz*dz) / dist (dv = v_j - v_i,
+ * 0 if dist == 0); counts[k] += 1, wsum[k] += |w_r|, wneg[k] += max(-w_r, 0).
+ * counts (nbins int64), wsum, wneg (nbins double): DEVICE, overwritten.
+ * 1 <= nbins <= 512. */
+int ld_pair_stats(ld_ctx* ctx, const double* rows, int64_t np, double rmax, int32_t nbins, int64_t* counts,
+                  double* wsum, double* wneg);
 /* Coalescence events for sorted pairs: visited in order, a pair merges only if
  * neither droplet merged earlier in this call; survivor i takes the summed
  * mass, mass-weighted velocity and position x_i + (m_j/m) d; row j is left

@@ -86,6 +86,7 @@ _SIGS = {
     "ld_collide_pairs": [_VP, _VP, _VP, ctypes.c_int64, ctypes.c_double, _VP, ctypes.c_int64,
                          ctypes.POINTER(ctypes.c_int64)],
     "ld_coalesce": [_VP, _VP, ctypes.c_int64, _VP, ctypes.c_int64, _VP, ctypes.POINTER(ctypes.c_int64)],
+    "ld_pair_stats": [_VP, _VP, ctypes.c_int64, ctypes.c_double, ctypes.c_int32, _VP, _VP, _VP],
 }
 EXPORTS = tuple(_SIGS) + ("ld_version",)
 

@@ -221,6 +221,14 @@ int ld_collide_pairs(ld_ctx* ctx, const double* rows, const double* radius, int6
   return ctx->eng->collide(rows, radius, np, rmax, pairs, max_pairs, npairs);
 }
 
+int ld_pair_stats(ld_ctx* ctx, const double* rows, int64_t np, double rmax, int32_t nbins, int64_t* counts,
+                  double* wsum, double* wneg) {
+  LD_CHECK_CTX(ctx); LD_NONNULL(counts); LD_NONNULL(wsum); LD_NONNULL(wneg);
+  if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
+  if (np > 1) LD_NONNULL(rows);
+  return ctx->eng->pair_stats(rows, np, rmax, nbins, counts, wsum, wneg);
+}
+
 int ld_coalesce(ld_ctx* ctx, double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
                 int64_t* nevents) {
   LD_CHECK_CTX(ctx); LD_NONNULL(nevents);

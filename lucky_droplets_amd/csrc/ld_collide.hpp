@@ -88,6 +88,74 @@ k_coll_pairs(const double* __restrict__ rows, const double* __restrict__ rad, in
       }
 }
 
+// Pair statistics within a cutoff (no reference counterpart; the input of the
+// radial distribution function and the kinematic collision kernel).  For every
+// pair i < j with d2 < rmax^2 (minimum image, the pair-test formula above):
+//   dist = sqrt(d2), bin k = min((int)(dist / dr), nbins - 1),
+//   w_r = ((dv_x d_x + dv_y d_y) + dv_z d_z) / dist  (dv = v_j - v_i; 0 if dist == 0),
+//   counts[k] += 1, wsum[k] += |w_r|, wneg[k] += max(-w_r, 0).
+// Per-block LDS histograms, then one global atomic per bin and block; counts
+// are exact, the FP64 sums depend on the summation order.
+static __global__ void __launch_bounds__(256)
+k_pair_stats(const double* __restrict__ rows, int np, double L, double cell, int nc, const int* __restrict__ start,
+             const int* __restrict__ order, const int* __restrict__ key, double rmax, double dr, int nbins,
+             unsigned long long* __restrict__ counts, double* __restrict__ wsum, double* __restrict__ wneg) {
+#pragma clang fp contract(off)
+  __shared__ unsigned long long hc[512];
+  __shared__ double hw[512], hn[512];
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x) { hc[k] = 0ull; hw[k] = 0.0; hn[k] = 0.0; }
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) {
+    const double* xi = rows + (size_t)i * 7;
+    double pi[3];
+    for (int a = 0; a < 3; ++a) pi[a] = wrapL(xi[a], L);
+    const double r2max = rmax * rmax;
+    const int ki = key[i];
+    const int cx = ki / (nc * nc), cy = (ki / nc) % nc, cz = ki % nc;
+    const int span = nc >= 3 ? 1 : 0;
+    const int ncells = nc * nc * nc;
+    for (int ox = -span; ox <= span; ++ox)
+      for (int oy = -span; oy <= span; ++oy)
+        for (int oz = -span; oz <= span; ++oz) {
+          const int c = ((((cx + ox) % nc + nc) % nc) * nc + (((cy + oy) % nc + nc) % nc)) * nc + (((cz + oz) % nc + nc) % nc);
+          const int b = start[c], e = (c + 1 < ncells) ? start[c + 1] : np;
+          for (int t = b; t < e; ++t) {
+            const int j = order[t];
+            if (j <= i) continue;
+            const double* xj = rows + (size_t)j * 7;
+            double d[3];
+            for (int a = 0; a < 3; ++a) {
+              double u = wrapL(xj[a], L) - pi[a];
+              u = u - L * rint(u / L);
+              d[a] = u;
+            }
+            const double d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+            if (!(d2 < r2max)) continue;
+            const double dist = sqrt(d2);
+            int k = (int)(dist / dr);
+            k = k < nbins ? k : nbins - 1;
+            double wr = 0.0;
+            if (dist > 0.0) {
+              const double dv0 = xj[3] - xi[3], dv1 = xj[4] - xi[4], dv2 = xj[5] - xi[5];
+              wr = ((dv0 * d[0] + dv1 * d[1]) + dv2 * d[2]) / dist;
+            }
+            atomicAdd(&hc[k], 1ull);
+            atomicAdd(&hw[k], fabs(wr));
+            if (wr < 0.0) atomicAdd(&hn[k], -wr);
+          }
+        }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x) {
+    if (hc[k]) {
+      atomicAdd(&counts[k], hc[k]);
+      unsafeAtomicAdd(&wsum[k], hw[k]);
+      unsafeAtomicAdd(&wneg[k], hn[k]);
+    }
+  }
+}
+
 // One compare-exchange step of a bitonic sort over n = 2^m keys.
 static __global__ void __launch_bounds__(256)
 k_bitonic(unsigned long long* __restrict__ a, int n, int j, int k) {

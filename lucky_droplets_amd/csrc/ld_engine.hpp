@@ -64,6 +64,10 @@ struct Engine final : EngineBase {
   static constexpr int P1 = zTileP<T, N, 1>(), P3 = zTileP<T, N, 3>(), P6 = zTileP<T, N, 6>();
   static constexpr int NBLK_Z1 = N * N / P1;       // blocks of a 1-field z pass
   static constexpr int NBLK_X = N * Nf / M;         // blocks of an x pass
+  static constexpr int M3 = XM3<T, N, M>::v;         // lines per block of a field-parallel x pass
+  static constexpr int NBLK_X3 = N * Nf / M3;
+  static constexpr int M4 = XM4<T, N, M>::v;         // lines per block of the four-field x pass
+  static constexpr int NBLK_X4 = N * Nf / M4;
   static constexpr int NBLK_SUM = 1024;             // blocks of k_sum2
 
   ld_params p;
@@ -329,7 +333,7 @@ struct Engine final : EngineBase {
   // u x omega in w6[0..3).
   int nonlinear(hipStream_t s, const C<T>* usp, T* uphys, V4<T>* g = nullptr) {
     LD_TRY(prof_mark(LD_K_XUW, s));
-    hipLaunchKernelGGL((k_x_uw<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, s, usp, w6, CF, (const C<T>*)tw, inv());
+    hipLaunchKernelGGL((k_x_uw3<T, N, M3>), dim3(NBLK_X3), dim3(3 * M3 * N1), 0, s, usp, w6, CF, (const C<T>*)tw, inv());
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XUW, s));
     OpZCross<T, N, P6> oc{w6, CF, uphys, g, inv()};
@@ -367,7 +371,7 @@ struct Engine final : EngineBase {
       LD_TRY((pass<1, -1>(s, wn, wn, 4, (T)1, x0, nx)));
     }
     LD_TRY(prof_mark(LD_K_XKN, s));
-    hipLaunchKernelGGL((k_x_kn<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, s, wn, CF, (const C<T>*)tw, inv());
+    hipLaunchKernelGGL((k_x_kn4<T, N, M4>), dim3(NBLK_X4), dim3(4 * M4 * N1), 0, s, wn, CF, (const C<T>*)tw, inv());
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XKN, s));
     for (int x0 = 0; x0 < N; x0 += nx) {
@@ -399,7 +403,7 @@ struct Engine final : EngineBase {
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
     ra.wacc = (T)wh; ra.anext = (T)anext; ra.scale = inv(); ra.acc_init = (s == 0);
     LD_TRY(prof_mark(LD_K_XRHS, st));
-    hipLaunchKernelGGL((k_x_rhs<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, ra, (const C<T>*)tw);
+    hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw);
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XRHS, st));
     // v, nv need the previous density chain done (n_s, clip, wn and v free)
@@ -596,7 +600,7 @@ struct Engine final : EngineBase {
     ra.nl = w6; ra.us = ukp; ra.ubase = ukp; ra.acc = nullptr; ra.usnext = nullptr; ra.ku = (C<T>*)ku;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit * visc); ra.tps = (T)p.tps;
     ra.wacc = (T)0; ra.anext = (T)0; ra.scale = inv(); ra.acc_init = 0;
-    hipLaunchKernelGGL((k_x_rhs<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, ra, (const C<T>*)tw);
+    hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw);
     LD_LAUNCHED();
     // the physical-space v the droplets see uses visc as well (forced-dns-sm.py:319-321)
     LD_TRY(velocity(st, vphys, nullptr, (const T*)nin, clip_none, nullptr));
@@ -809,17 +813,13 @@ struct Engine final : EngineBase {
 
   static int64_t pow2ceil(int64_t x) { int64_t v = 2; while (v < x) v <<= 1; return v; }
 
-  int collide(const double* rows, const double* rad, int64_t n_, double rmax, int64_t* pairs, int64_t max_pairs,
-              int64_t* count) override {
-    *count = 0;
-    if (n_ < 2) return LD_OK;
-    if (n_ > 0x7fffffff) { set_error("too many droplets"); return LD_ERR_ARG; }
-    LD_TRY(join(st, st2, evB));
-    // cells of side >= 2 rmax; fewer than 3 per side -> one cell (all pairs)
-    int nc = (rmax > 0.0) ? (int)floor(p.L / (2.0 * rmax)) : 256;
+  // Cell list of the droplet rows on st: cells of side >= reach (fewer than 3
+  // per side -> one cell, all pairs); bin starts in chist, slots in corder.
+  int build_cells(const double* rows, int64_t n_, double reach, int& nc, double& cell) {
+    nc = (reach > 0.0) ? (int)floor(p.L / reach) : 256;
     if (nc > 256) nc = 256;
     if (nc < 3) nc = 1;
-    const double cell = p.L / nc;
+    cell = p.L / nc;
     const int ncells = nc * nc * nc;
     if (n_ > ccap) {
       LD_TRY(alloc(ckey, n_)); LD_TRY(alloc(crank, n_)); LD_TRY(alloc(corder, n_)); LD_TRY(alloc(cused, n_));
@@ -829,9 +829,6 @@ struct Engine final : EngineBase {
       LD_TRY(alloc(chist, ncells)); LD_TRY(alloc(cbsum, (ncells + 4095) / 4096));
       chist_cap = ncells;
     }
-    const int64_t kc = pow2ceil(max_pairs > 2 ? max_pairs : 2);
-    if (kc > ckcap) { LD_TRY(alloc(ckeys, kc)); ckcap = kc; }
-    if (!ccount) { LD_TRY(alloc(ccount, 1)); LD_TRY(alloc(cnev, 1)); }
     const unsigned pb = (unsigned)((n_ + 255) / 256);
     LD_TRY(fill<int>(st, chist, ncells, 0));
     hipLaunchKernelGGL(k_coll_count, dim3(pb), dim3(256), 0, st, rows, (int)n_, p.L, cell, nc, ckey, crank, chist);
@@ -846,6 +843,43 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL(k_coll_scatter, dim3(pb), dim3(256), 0, st, (int)n_, (const int*)ckey, (const int*)crank,
                        (const int*)chist, corder);
     LD_LAUNCHED();
+    return LD_OK;
+  }
+
+  int pair_stats(const double* rows, int64_t n_, double rmax, int nbins, int64_t* counts, double* wsum,
+                 double* wneg) override {
+    if (nbins < 1 || nbins > 512) { set_error("nbins must be in [1, 512]"); return LD_ERR_ARG; }
+    if (!(rmax > 0.0) || rmax > 0.5 * p.L) { set_error("rmax must be in (0, L/2]"); return LD_ERR_ARG; }
+    if (n_ > 0x7fffffff) { set_error("too many droplets"); return LD_ERR_ARG; }
+    LD_TRY(join(st, st2, evB));
+    LD_TRY(fill<unsigned long long>(st, (unsigned long long*)counts, (size_t)nbins, 0ull));
+    LD_TRY(fill<double>(st, wsum, (size_t)nbins, 0.0));
+    LD_TRY(fill<double>(st, wneg, (size_t)nbins, 0.0));
+    if (n_ < 2) return LD_OK;
+    int nc;
+    double cell;
+    LD_TRY(build_cells(rows, n_, rmax, nc, cell));
+    hipLaunchKernelGGL(k_pair_stats, dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, rows, (int)n_, p.L, cell,
+                       nc, (const int*)chist, (const int*)corder, (const int*)ckey, rmax, rmax / nbins, nbins,
+                       (unsigned long long*)counts, wsum, wneg);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+
+  int collide(const double* rows, const double* rad, int64_t n_, double rmax, int64_t* pairs, int64_t max_pairs,
+              int64_t* count) override {
+    *count = 0;
+    if (n_ < 2) return LD_OK;
+    if (n_ > 0x7fffffff) { set_error("too many droplets"); return LD_ERR_ARG; }
+    LD_TRY(join(st, st2, evB));
+    // cells of side >= 2 rmax (the largest pair distance)
+    int nc;
+    double cell;
+    LD_TRY(build_cells(rows, n_, 2.0 * rmax, nc, cell));
+    const int64_t kc = pow2ceil(max_pairs > 2 ? max_pairs : 2);
+    if (kc > ckcap) { LD_TRY(alloc(ckeys, kc)); ckcap = kc; }
+    if (!ccount) { LD_TRY(alloc(ccount, 1)); LD_TRY(alloc(cnev, 1)); }
+    const unsigned pb = (unsigned)((n_ + 255) / 256);
     LD_TRY(fill<unsigned long long>(st, ccount, 1, 0ull));
     hipLaunchKernelGGL(k_coll_pairs, dim3(pb), dim3(256), 0, st, rows, rad, (int)n_, p.L, cell, nc,
                        (const int*)chist, (const int*)corder, (const int*)ckey, ckeys,

@@ -29,6 +29,8 @@ struct EngineBase {
   virtual int exterp(const double* coord, int64_t np, const double* w, void* c) = 0;
   virtual int collide(const double* rows, const double* rad, int64_t np, double rmax, int64_t* pairs,
                       int64_t max_pairs, int64_t* count) = 0;
+  virtual int pair_stats(const double* rows, int64_t np, double rmax, int nbins, int64_t* counts, double* wsum,
+                         double* wneg) = 0;
   virtual int coalesce(double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
                        int64_t* nevents) = 0;
   virtual int spectrum(const void* f, int nf, double w, double* out) = 0;

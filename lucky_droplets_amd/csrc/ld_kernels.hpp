@@ -418,56 +418,6 @@ struct OpZKn {
 };
 
 // ======================================================== fused x passes
-// Velocity and vorticity (forced-dns-sm.py:291-297): x-inverse of u_hat and of
-// i k x u_hat.  Output: 6 fields (u_x,u_y,u_z,w_x,w_y,w_z), x-inverse-transformed.
-template <typename T, int N, int M>
-__global__ void __launch_bounds__(M * Plan<N>::N1)
-k_x_uw(const C<T>* __restrict__ uk, C<T>* __restrict__ out, size_t fs, const C<T>* __restrict__ tw, T scale) {
-  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
-  __shared__ C<T> lds[M * (N + 1)];
-  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
-  const int q = blockIdx.x * M + m;
-  const T ky = (T)kwave(q / Nf, N), kz = (T)(q % Nf);
-  C<T>* L = lds + m * (N + 1);
-  C<T> u0[N2], u1[N2], u2[N2], a[N2];
-#pragma unroll
-  for (int r = 0; r < N2; ++r) {
-    const uint32_t o = (uint32_t)(q + posOut<N>(n1, r) * js);
-    u0[r] = ld32(uk, o); u1[r] = ld32(uk + fs, o); u2[r] = ld32(uk + 2 * fs, o);
-  }
-  auto store = [&](int comp, const C<T>* v) {
-    C<T>* O = out + comp * fs;
-#pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) st32(O, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(v[n2], scale));
-  };
-  // vorticity i k x u first (all three components of u are live), then u in place
-#pragma unroll
-  for (int r = 0; r < N2; ++r)
-    a[r] = cmuli<T>(mkc<T>(ky * u2[r].x - kz * u1[r].x, ky * u2[r].y - kz * u1[r].y));
-  lineInv<T, N>(a, L, n1, true, tw);
-  store(3, a);
-#pragma unroll
-  for (int r = 0; r < N2; ++r) {
-    const T kx = (T)kwave(posOut<N>(n1, r), N);
-    a[r] = cmuli<T>(mkc<T>(kz * u0[r].x - kx * u2[r].x, kz * u0[r].y - kx * u2[r].y));
-  }
-  lineInv<T, N>(a, L, n1, true, tw);
-  store(4, a);
-#pragma unroll
-  for (int r = 0; r < N2; ++r) {
-    const T kx = (T)kwave(posOut<N>(n1, r), N);
-    a[r] = cmuli<T>(mkc<T>(kx * u1[r].x - ky * u0[r].x, kx * u1[r].y - ky * u0[r].y));
-  }
-  lineInv<T, N>(a, L, n1, true, tw);
-  store(5, a);
-  lineInv<T, N>(u0, L, n1, true, tw);
-  store(0, u0);
-  lineInv<T, N>(u1, L, n1, true, tw);
-  store(1, u1);
-  lineInv<T, N>(u2, L, n1, true, tw);
-  store(2, u2);
-}
-
 // Arguments of the fused spectral RHS pass.
 template <typename T> struct RhsArgs {
   C<T>* nl;            // in: u x w, forward-transformed along z and y (3 fields); out: v_hat x-inverse
@@ -482,34 +432,97 @@ template <typename T> struct RhsArgs {
   int acc_init;
 };
 
+// ---------------------------------------------- field-parallel x passes
+// The kernels below give each of the three vector components its own thread
+// group (3 * M * N1 threads): every thread transforms one field, so a thread
+// holds N2 elements instead of 3 * N2, and the components meet in LDS for the
+// pointwise work.  Lines per block M3 keeps the three LDS regions <= 100 KB.
+template <typename T, int N, int M0> struct XM3 {
+  static constexpr int v = (3L * M0 * (N + 1) * (long)sizeof(C<T>) <= 102400L) ? M0 : M0 / 2;
+};
+
+// Velocity and vorticity (forced-dns-sm.py:291-297): x-inverse of u_hat and of
+// i k x u_hat, 6 output fields (u_x, u_y, u_z, w_x, w_y, w_z).  Group f
+// inverse-transforms u_f and w_f.
+template <typename T, int N, int M>
+__global__ void __launch_bounds__(3 * M * Plan<N>::N1)
+k_x_uw3(const C<T>* __restrict__ uk, C<T>* __restrict__ out, size_t fs, const C<T>* __restrict__ tw, T scale) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
+  __shared__ C<T> lds[3 * M * (N + 1)];
+  const int f = threadIdx.x / G, loc = threadIdx.x % G;
+  const int m = loc % M, n1 = loc / M;
+  const int q = blockIdx.x * M + m;
+  const T ky = (T)kwave(q / Nf, N), kz = (T)(q % Nf);
+  C<T>* L[3] = {lds + m * (N + 1), lds + (M + m) * (N + 1), lds + (2 * M + m) * (N + 1)};
+  C<T>* Lf = lds + (f * M + m) * (N + 1);
+  C<T> u[N2], w[N2];
+#pragma unroll
+  for (int r = 0; r < N2; ++r) {
+    u[r] = ld32(uk + f * fs, (uint32_t)(q + posOut<N>(n1, r) * js));
+    Lf[posOut<N>(n1, r)] = u[r];
+  }
+  __syncthreads();
+  // vorticity component f = i (k x u)_f  (forced-dns-sm.py:294-296)
+#pragma unroll
+  for (int r = 0; r < N2; ++r) {
+    const int ix = posOut<N>(n1, r);
+    const T kx = (T)kwave(ix, N);
+    if (f == 0) {
+      const C<T> u1 = L[1][ix], u2 = L[2][ix];
+      w[r] = cmuli<T>(mkc<T>(ky * u2.x - kz * u1.x, ky * u2.y - kz * u1.y));
+    } else if (f == 1) {
+      const C<T> u0 = L[0][ix], u2 = L[2][ix];
+      w[r] = cmuli<T>(mkc<T>(kz * u0.x - kx * u2.x, kz * u0.y - kx * u2.y));
+    } else {
+      const C<T> u0 = L[0][ix], u1 = L[1][ix];
+      w[r] = cmuli<T>(mkc<T>(kx * u1.x - ky * u0.x, kx * u1.y - ky * u0.y));
+    }
+  }
+  // (lineInv synchronises before it reuses the regions)
+  lineInv<T, N>(w, Lf, n1, true, tw);
+  C<T>* Ow = out + (3 + f) * fs;
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) st32(Ow, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(w[n2], scale));
+  lineInv<T, N>(u, Lf, n1, true, tw);
+  C<T>* Ou = out + f * fs;
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) st32(Ou, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(u[n2], scale));
+}
+
 // Nonlinear term + forcing + dealiasing + pressure projection + viscosity
 // (forced-dns-sm.py:239-275, 304-316), the RK4 accumulation of u_hat
 // (:338-360) and the slow-manifold velocity v_hat (:319-321) x-inverse
 // transformed for the next passes.
+// Group f forward-transforms nl_f, the pointwise stage runs on the LDS
+// spectra (modes r = f, f+3, ... of each thread's line), then group f
+// inverse-transforms v_f.
 template <typename T, int N, int M>
-__global__ void __launch_bounds__(M * Plan<N>::N1)
-k_x_rhs(RhsArgs<T> A, const C<T>* __restrict__ tw) {
-  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
-  __shared__ C<T> lds[M * (N + 1)];
-  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
+__global__ void __launch_bounds__(3 * M * Plan<N>::N1)
+k_x_rhs3(RhsArgs<T> A, const C<T>* __restrict__ tw) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
+  __shared__ C<T> lds[3 * M * (N + 1)];
+  const int f = threadIdx.x / G, loc = threadIdx.x % G;
+  const int m = loc % M, n1 = loc / M;
   const int q = blockIdx.x * M + m;
   const int iy = q / Nf, iz = q % Nf;
-  C<T>* L = lds + m * (N + 1);
+  C<T>* L0 = lds + m * (N + 1);
+  C<T>* L1 = lds + (M + m) * (N + 1);
+  C<T>* L2 = lds + (2 * M + m) * (N + 1);
+  C<T>* Lf = lds + (f * M + m) * (N + 1);
   const size_t fs = A.fs;
-  C<T>* NL0 = A.nl; C<T>* NL1 = A.nl + fs; C<T>* NL2 = A.nl + 2 * fs;
+  C<T>* NLf = A.nl + f * fs;
+  C<T> a[N2];
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js));
+  lineFwd<T, N>(a, Lf, n1, true, tw);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < N2; ++r) Lf[posOut<N>(n1, r)] = a[r];
+  __syncthreads();
   const C<T>* US0 = A.us; const C<T>* US1 = A.us + fs; const C<T>* US2 = A.us + 2 * fs;
   const C<T>* UB0 = A.ubase; const C<T>* UB1 = A.ubase + fs; const C<T>* UB2 = A.ubase + 2 * fs;
-  C<T> a0[N2], a1[N2], a2[N2];
-#pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) {
-    const uint32_t o = (uint32_t)(q + posIn<N>(n1, n2) * js);
-    a0[n2] = ld32(NL0, o); a1[n2] = ld32(NL1, o); a2[n2] = ld32(NL2, o);
-  }
-  lineFwd<T, N>(a0, L, n1, true, tw);
-  lineFwd<T, N>(a1, L, n1, true, tw);
-  lineFwd<T, N>(a2, L, n1, true, tw);
-#pragma unroll
-  for (int r = 0; r < N2; ++r) {
+#pragma unroll 2
+  for (int r = f; r < N2; r += 3) {
     const int ix = posOut<N>(n1, r);
     const Mode<T> md = mode<T, N>(ix, iy, iz);
     const uint32_t o = (uint32_t)(q + ix * js);
@@ -523,9 +536,9 @@ k_x_rhs(RhsArgs<T> A, const C<T>* __restrict__ tw) {
     f1 = cscale<T>(cadd<T>(f1, cscale<T>(pf, md.ky)), md.mask);
     f2 = cscale<T>(cadd<T>(f2, cscale<T>(pf, md.kz)), md.mask);
     // dealiased nonlinear term + forcing (:304-306)
-    const C<T> R0 = cscale<T>(cadd<T>(a0[r], f0), md.mask);
-    const C<T> R1 = cscale<T>(cadd<T>(a1[r], f1), md.mask);
-    const C<T> R2 = cscale<T>(cadd<T>(a2[r], f2), md.mask);
+    const C<T> R0 = cscale<T>(cadd<T>(L0[ix], f0), md.mask);
+    const C<T> R1 = cscale<T>(cadd<T>(L1[ix], f1), md.mask);
+    const C<T> R2 = cscale<T>(cadd<T>(L2[ix], f2), md.mask);
     // pressure pk = i invlap (k . R)  (:309)
     const C<T> D = mkc<T>(md.kx * R0.x + md.ky * R1.x + md.kz * R2.x, md.kx * R0.y + md.ky * R1.y + md.kz * R2.y);
     const C<T> pk = cmuli<T>(cscale<T>(D, md.invlap));
@@ -552,59 +565,71 @@ k_x_rhs(RhsArgs<T> A, const C<T>* __restrict__ tw) {
       st32(A.usnext + 2 * fs, o, cadd<T>(b2, cscale<T>(k2, A.anext)));
     }
     // v_hat = (u - tps (-i k p - nu k^2 u)) * mask  (:319-321)
-    a0[r] = cscale<T>(csub<T>(us0, cscale<T>(csub<T>(cscale<T>(ikp0, (T)-1), vs0), A.tps)), md.mask);
-    a1[r] = cscale<T>(csub<T>(us1, cscale<T>(csub<T>(cscale<T>(ikp1, (T)-1), vs1), A.tps)), md.mask);
-    a2[r] = cscale<T>(csub<T>(us2, cscale<T>(csub<T>(cscale<T>(ikp2, (T)-1), vs2), A.tps)), md.mask);
-    __builtin_amdgcn_sched_barrier(0);
+    L0[ix] = cscale<T>(csub<T>(us0, cscale<T>(csub<T>(cscale<T>(ikp0, (T)-1), vs0), A.tps)), md.mask);
+    L1[ix] = cscale<T>(csub<T>(us1, cscale<T>(csub<T>(cscale<T>(ikp1, (T)-1), vs1), A.tps)), md.mask);
+    L2[ix] = cscale<T>(csub<T>(us2, cscale<T>(csub<T>(cscale<T>(ikp2, (T)-1), vs2), A.tps)), md.mask);
   }
-  lineInv<T, N>(a0, L, n1, true, tw);
-  lineInv<T, N>(a1, L, n1, true, tw);
-  lineInv<T, N>(a2, L, n1, true, tw);
+  __syncthreads();
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) {
-    const uint32_t o = (uint32_t)(q + posIn<N>(n1, n2) * js);
-    st32(NL0, o, cscale<T>(a0[n2], A.scale));
-    st32(NL1, o, cscale<T>(a1[n2], A.scale));
-    st32(NL2, o, cscale<T>(a2[n2], A.scale));
-  }
+  for (int r = 0; r < N2; ++r) a[r] = Lf[posOut<N>(n1, r)];
+  lineInv<T, N>(a, Lf, n1, true, tw);
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
 }
 
 // Density RHS in spectral space (forced-dns-sm.py:325-330):
 //   kn_hat = (-i k.(nv)_hat*mask - fc_hat*mask) * mask, then x-inverse.
 // In: 4 fields (nv_x, nv_y, nv_z, fc) forward-transformed along z and y;
 // out: field 0 (x-inverse of kn_hat).
+// Four field groups (4 * M * N1 threads) forward-transform one input each;
+// the pointwise stage runs on the LDS spectra (modes r = g, g+4, ... of each
+// thread's line) and group 0 inverse-transforms the result.  Lines per block
+// keep the four LDS regions <= 132 KB.
+template <typename T, int N, int M0> struct XM4 {
+  static constexpr int v = (4L * M0 * (N + 1) * (long)sizeof(C<T>) <= 135168L) ? M0 : M0 / 2;
+};
 template <typename T, int N, int M>
-__global__ void __launch_bounds__(M * Plan<N>::N1)
-k_x_kn(C<T>* w, size_t fs, const C<T>* __restrict__ tw, T scale) {
-  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
-  __shared__ C<T> lds[M * (N + 1)];
-  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
+__global__ void __launch_bounds__(4 * M * Plan<N>::N1)
+k_x_kn4(C<T>* w, size_t fs, const C<T>* __restrict__ tw, T scale) {
+  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
+  __shared__ C<T> lds[4 * M * (N + 1)];
+  const int g = threadIdx.x / G, loc = threadIdx.x % G;
+  const int m = loc % M, n1 = loc / M;
   const int q = blockIdx.x * M + m;
   const int iy = q / Nf, iz = q % Nf;
-  C<T>* L = lds + m * (N + 1);
-  C<T> acc[N2], a[N2];
-#pragma unroll 1
-  for (int f = 0; f < 4; ++f) {
-    const C<T>* W = w + f * fs;
+  C<T>* L0 = lds + m * (N + 1);
+  C<T>* L1 = lds + (M + m) * (N + 1);
+  C<T>* L2 = lds + (2 * M + m) * (N + 1);
+  C<T>* L3 = lds + (3 * M + m) * (N + 1);
+  C<T>* Lg = lds + (g * M + m) * (N + 1);
+  C<T> a[N2];
+  const C<T>* W = w + g * fs;
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(W, (uint32_t)(q + posIn<N>(n1, n2) * js));
-    lineFwd<T, N>(a, L, n1, true, tw);
+  for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(W, (uint32_t)(q + posIn<N>(n1, n2) * js));
+  lineFwd<T, N>(a, Lg, n1, true, tw);
+  __syncthreads();
 #pragma unroll
-    for (int r = 0; r < N2; ++r) {
-      const Mode<T> md = mode<T, N>(posOut<N>(n1, r), iy, iz);
-      const C<T> v = cscale<T>(a[r], md.mask);
-      if (f == 0) acc[r] = cscale<T>(v, md.kx);
-      else if (f == 1) acc[r] = cadd<T>(acc[r], cscale<T>(v, md.ky));
-      else if (f == 2) acc[r] = cadd<T>(acc[r], cscale<T>(v, md.kz));
-      else {
-        const C<T> mi = mkc<T>(acc[r].y, -acc[r].x);   // -i * (k . nv)
-        acc[r] = cscale<T>(csub<T>(mi, v), md.mask);
-      }
-    }
+  for (int r = 0; r < N2; ++r) Lg[posOut<N>(n1, r)] = a[r];
+  __syncthreads();
+  for (int r = g; r < N2; r += 4) {
+    const int ix = posOut<N>(n1, r);
+    const Mode<T> md = mode<T, N>(ix, iy, iz);
+    C<T> acc = cscale<T>(cscale<T>(L0[ix], md.mask), md.kx);
+    acc = cadd<T>(acc, cscale<T>(cscale<T>(L1[ix], md.mask), md.ky));
+    acc = cadd<T>(acc, cscale<T>(cscale<T>(L2[ix], md.mask), md.kz));
+    const C<T> mi = mkc<T>(acc.y, -acc.x);   // -i * (k . nv)
+    L0[ix] = cscale<T>(csub<T>(mi, cscale<T>(L3[ix], md.mask)), md.mask);
   }
-  lineInv<T, N>(acc, L, n1, true, tw);
+  __syncthreads();
+  if (g == 0) {
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) st32(w, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(acc[n2], scale));
+    for (int r = 0; r < N2; ++r) a[r] = L0[posOut<N>(n1, r)];
+  }
+  lineInv<T, N>(a, L0, n1, g == 0, tw);
+  if (g == 0) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) st32(w, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
+  }
 }
 
 // Last pass of the reality round trip + solenoidal projection

@@ -129,6 +129,34 @@ class MPI_particles:
                                             ctypes.byref(cnt)))
         return out[: cnt.value].clone()
 
+    def pair_statistics(self, rmax, nbins=64):
+        """Pair statistics of the current rows within the cutoff ``rmax``:
+        separation histogram, radial relative velocity sums, and from them the
+        radial distribution function g(r), the mean |w_r| and the kinematic
+        collision kernel K(r) = 2 pi r^2 g(r) <|w_r|>(r) at the bin centres
+        (Sundaram & Collins 1997).  Returns a dict of numpy arrays."""
+        dns = self.dns
+        coord, npart = dns._rows(self.coord)
+        counts = torch.empty((nbins,), dtype=torch.int64, device=dns.device)
+        wsum = torch.empty((nbins,), dtype=torch.float64, device=dns.device)
+        wneg = torch.empty((nbins,), dtype=torch.float64, device=dns.device)
+        dns._stream()
+        _lib.check(dns.lib.ld_pair_stats(dns.ctx, _ptr(coord), npart, float(rmax), int(nbins), _ptr(counts),
+                                         _ptr(wsum), _ptr(wneg)))
+        c = counts.cpu().numpy()
+        ws = wsum.cpu().numpy()
+        wn = wneg.cpu().numpy()
+        edges = np.arange(nbins + 1) * (rmax / nbins)
+        centre = 0.5 * (edges[1:] + edges[:-1])
+        shell = 4.0 / 3.0 * math.pi * (edges[1:] ** 3 - edges[:-1] ** 3)
+        pairs_per_volume = npart * (npart - 1) / 2.0 / self.L ** 3
+        with np.errstate(invalid="ignore", divide="ignore"):
+            g = c / (pairs_per_volume * shell) if npart > 1 else np.zeros(nbins)
+            wr = np.where(c > 0, ws / np.maximum(c, 1), np.nan)
+            wneg_mean = np.where(c > 0, wn / np.maximum(c, 1), np.nan)
+        return {"edges": edges, "counts": c, "wsum": ws, "wneg": wn, "g": g, "wr_abs_mean": wr,
+                "wr_in_mean": wneg_mean, "kernel": 2.0 * math.pi * centre ** 2 * g * wr}
+
     def coalesce(self, pairs):
         """Apply the greedy merge rule in place on coord; returns the (i, j) events."""
         dns = self.dns

@@ -49,6 +49,39 @@ def pairs_bruteforce(pos: np.ndarray, radius: np.ndarray, L: float) -> np.ndarra
     return np.array(out, dtype=np.int64).reshape(-1, 2)
 
 
+def pair_stats_bruteforce(rows: np.ndarray, L: float, rmax: float, nbins: int):
+    """Pair statistics within rmax, O(Np^2): for every pair i < j with
+    minimum-image d2 < rmax^2 (d2 as in pairs_bruteforce),
+        dist = sqrt(d2), k = min(int(dist / (rmax / nbins)), nbins - 1),
+        w_r = ((dvx*dx + dvy*dy) + dvz*dz) / dist   (dv = v_j - v_i; 0 if dist == 0),
+    returns counts[k] (int64), wsum[k] = sum |w_r|, wneg[k] = sum max(-w_r, 0)."""
+    rows = np.asarray(rows, dtype=np.float64)
+    x = wrap(rows[:, :3], L)
+    v = rows[:, 3:6]
+    n = x.shape[0]
+    dr = rmax / nbins
+    counts = np.zeros(nbins, dtype=np.int64)
+    wsum = np.zeros(nbins)
+    wneg = np.zeros(nbins)
+    for i in range(n - 1):
+        d = x[i + 1:] - x[i]
+        d = d - L * np.rint(d / L)
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        hit = np.nonzero(d2 < rmax * rmax)[0]
+        if hit.size == 0:
+            continue
+        dist = np.sqrt(d2[hit])
+        k = np.minimum((dist / dr).astype(np.int64), nbins - 1)
+        dv = v[i + 1:][hit] - v[i]
+        dh = d[hit]
+        num = (dv[:, 0] * dh[:, 0] + dv[:, 1] * dh[:, 1]) + dv[:, 2] * dh[:, 2]
+        wr = np.where(dist > 0, num / np.where(dist > 0, dist, 1.0), 0.0)
+        np.add.at(counts, k, 1)
+        np.add.at(wsum, k, np.abs(wr))
+        np.add.at(wneg, k, np.maximum(-wr, 0.0))
+    return counts, wsum, wneg
+
+
 def coalesce(rows: np.ndarray, pairs: np.ndarray, L: float):
     """Apply the greedy merge rule to (Np, 7) rows; returns (rows, events)."""
     rows = np.array(rows, dtype=np.float64, copy=True)

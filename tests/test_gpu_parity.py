@@ -275,6 +275,35 @@ def test_collision_capacity_error(gpu):
     d.close()
 
 
+@pytest.mark.parametrize("n,rmax,nbins,seed", [(4000, 0.6, 48, 0), (1500, 2.5, 20, 1), (3000, 0.2, 512, 2)])
+def test_pair_statistics(gpu, n, rmax, nbins, seed):
+    """Separation histogram bit-exact, radial-velocity sums to 1e-12, g(r) and
+    the collision kernel derived from them (rmax = 2.5 > L/3 takes the
+    one-cell path)."""
+    from lucky_droplets_amd.dns import ForcedDNS
+    from lucky_droplets_amd.particles import MPI_particles
+    from oracle import collide
+    cfg = _cfg(32, "f64", n)
+    rng = np.random.default_rng(seed)
+    rows = rng.uniform(0, cfg.L, (n, 7))
+    rows[:, :3] += rng.integers(-2, 3, (n, 3)) * cfg.L
+    rows[:, 3:6] = rng.normal(0, 1, (n, 3))
+    d = ForcedDNS(cfg)
+    stb = MPI_particles(None, cfg.L, cfg.N, n, cfg.sts, 0.05, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0,
+                        dns=d, coord=rows)
+    got = stb.pair_statistics(rmax, nbins)
+    c, ws, wn = collide.pair_stats_bruteforce(rows, cfg.L, rmax, nbins)
+    assert c.sum() > 100
+    assert np.array_equal(got["counts"], c)
+    assert np.allclose(got["wsum"], ws, rtol=1e-12, atol=0)
+    assert np.allclose(got["wneg"], wn, rtol=1e-12, atol=0)
+    # uniform random positions: g(r) ~ 1 where the bins hold many pairs
+    full = got["counts"] > 2000
+    if full.any():
+        assert np.all(np.abs(got["g"][full] - 1.0) < 0.15)
+    d.close()
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_diagnostics(gpu, prec):
     """E(k), the n spectrum, Pi(k) and field statistics of save() / plot.py."""

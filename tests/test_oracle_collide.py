@@ -46,3 +46,49 @@ def test_coalescence_conserves_mass_and_momentum():
     assert np.allclose((out[:, 3:6] * out[:, 6:7]).sum(0), mom, rtol=1e-13, atol=1e-13)
     assert out[3, 6] == 0 and out[9, 6] == 0 and out[13, 6] == 0
     assert np.all((out[:, :3] >= 0) & (out[:, :3] < L))
+
+
+def test_pair_stats_lattice_shells():
+    """Simple cubic lattice of spacing a: every point has 6, 12, 8, 6
+    neighbours at a, sqrt(2) a, sqrt(3) a, 2 a."""
+    m = 6
+    a = L / m
+    g = (np.arange(m) + 0.25) * a
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    rows = np.zeros((pos.shape[0], 7))
+    rows[:, :3] = pos
+    nbins = 40
+    rmax = 2.05 * a
+    c, ws, wn = collide.pair_stats_bruteforce(rows, L, rmax, nbins)
+    n = pos.shape[0]
+    dr = rmax / nbins
+    for dist, per in [(a, 6), (np.sqrt(2) * a, 12), (np.sqrt(3) * a, 8), (2 * a, 6)]:
+        assert c[int(dist / dr)] == n * per // 2
+    assert c.sum() == n * 32 // 2
+    assert np.all(ws == 0) and np.all(wn == 0)          # no relative motion
+
+
+def test_pair_stats_uniform_expansion():
+    """v = alpha x on a cluster far from the seams: w_r = alpha |d| for every
+    pair; contraction gives wneg = wsum."""
+    rng = np.random.default_rng(3)
+    pos = 2.0 + rng.uniform(0, 1.0, (300, 3))
+    for alpha in (0.7, -0.4):
+        rows = np.zeros((300, 7))
+        rows[:, :3] = pos
+        rows[:, 3:6] = alpha * pos
+        c, ws, wn = collide.pair_stats_bruteforce(rows, L, 2.0, 16)
+        d = pos[:, None, :] - pos[None, :, :]
+        dist = np.sqrt((d ** 2).sum(-1))[np.triu_indices(300, 1)]
+        assert c.sum() == dist.size
+        assert np.isclose(ws.sum(), abs(alpha) * dist.sum(), rtol=1e-12)
+        assert np.isclose(wn.sum(), (-alpha if alpha < 0 else 0.0) * dist.sum(), rtol=1e-12, atol=1e-12)
+
+
+def test_pair_stats_matches_pair_test():
+    """Cutoff rmax counts exactly the pairs of radius rmax/2 each."""
+    rng = np.random.default_rng(4)
+    rows = rng.uniform(0, L, (500, 7))
+    rmax = 0.7
+    c, _, _ = collide.pair_stats_bruteforce(rows, L, rmax, 10)
+    assert c.sum() == collide.pairs_bruteforce(rows[:, :3], np.full(500, rmax / 2), L).shape[0]
