@@ -329,9 +329,10 @@ k_particles_stage(PStage<T, PT> A) {
 
 // ------------------------------------------------------------ brick deposit
 // Deposit of W with the cosine kernel / dx^3 (particles.py:23-35, 529-575),
-// organised by output and free of atomics: one workgroup per brick of S^3
-// grid nodes; thread (y, z) owns the node column x = 0..S-1 of the brick in
-// registers and writes it once.
+// organised by output: one workgroup per brick of S^3 grid nodes accumulates
+// in an FP64 LDS tile (ds_add_f64 runs several times faster than ds_add_f32
+// on gfx950, and the sum keeps FP64 precision) and writes each node of the
+// brick exactly once.
 //
 // The droplets that can reach the brick are those whose stage cell lies in
 // [c0 - 2, c0 + S] per axis; their sorted bins (bins of the position at the
@@ -340,47 +341,41 @@ k_particles_stage(PStage<T, PT> A) {
 // each (bin x, bin y) of that window the z-run of bins is one contiguous range
 // of sorted slots per bin tile it crosses, so the candidates form at most
 // NZS * NBX^2 ranges.  Per round, CH candidates are tested for overlap with the
-// brick; those that overlap are compacted into an LDS list with their stencil
-// origin and weights, and every wavefront then walks the list, skipping
-// droplets that miss its rows.
+// brick; those that overlap are compacted into an LDS list, and each listed
+// droplet is then deposited by one lane.
 template <int N> struct Brick {
   static constexpr int S = Bins<N>::TC;              // nodes per side (one bin tile)
   static constexpr int NB = N / S;                   // bricks per side
-  static constexpr int NT = S * S;                   // threads: one per (y, z) column
+  static constexpr int NT = 256;                     // threads
   // bins spanned by S + 3 stage cells from a bin-aligned start, +-1 bin of motion
   static constexpr int W0 = (S + 2 + Bins<N>::BS - 1) / Bins<N>::BS + 1 + 2;
   static constexpr int NBX = W0 < Bins<N>::B ? W0 : Bins<N>::B;
   // z-runs of NBX bins split at tile boundaries into at most NZS key ranges
   static constexpr int NZS = (NBX + Bins<N>::TS - 1) / Bins<N>::TS + 1;
   static constexpr int NR = NZS * NBX * NBX;         // slot ranges
+  static constexpr int CH = 4 * NT;                  // candidates tested per round
   static_assert(N % S == 0 && (NB * NB * NB) % 8 == 0, "brick grid");
 };
-
-// Packed brick-relative stencil origin (each in [-3, S-1]).
-__device__ __forceinline__ int packOrg(int rx, int ry, int rz) { return (rx + 3) | ((ry + 3) << 8) | ((rz + 3) << 16); }
 
 template <typename T, int N>
 __global__ void __launch_bounds__(Brick<N>::NT)
 k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, int np, T* __restrict__ fc,
                 DropPhys P) {
   using Bk = Brick<N>;
-  constexpr int S = Bk::S, NBr = Bk::NB, NBX = Bk::NBX, NR = Bk::NR, NT = Bk::NT, NZS = Bk::NZS;
+  constexpr int S = Bk::S, NBr = Bk::NB, NBX = Bk::NBX, NR = Bk::NR, NT = Bk::NT, NZS = Bk::NZS, CH = Bk::CH;
   constexpr int TS = Bins<N>::TS;
-  constexpr int NW = NT / 64;                               // wavefronts
-  constexpr int RPW = 64 / S;                               // brick rows (y) per wavefront
-  constexpr int CH = (sizeof(T) == 4 ? 4 : 2) * NT;         // candidates per round
   constexpr int BS = Bins<N>::BS, B = Bins<N>::B;
   constexpr int NBIN = B * B * B;
+  __shared__ double tile[S * S * S];
   __shared__ int rb[NR], rn[NR], rl[NR + 1];
-  __shared__ int lorg[CH];
-  __shared__ V4<T> lwx[CH];      // W inv_vol * w_x(a), a = 0..3
-  __shared__ V4<T> lyz[CH];      // sin, cos of the y and z stencil angles
+  __shared__ int list[CH];
   __shared__ int cnt;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
   // XCD-aware: each of the 8 XCDs takes a contiguous run of bricks
   const int nb = gridDim.x, per = nb / 8;
   const int br = (blockIdx.x % 8) * per + blockIdx.x / 8;
   const int c0x = (br / (NBr * NBr)) * S, c0y = ((br / NBr) % NBr) * S, c0z = (br % NBr) * S;
+  for (int i = tid; i < S * S * S; i += NT) tile[i] = 0.0;
   if (tid == 0) cnt = 0;
   // slot ranges of the window's z-runs
   const int lo = (NBX == B) ? 0 : 1;
@@ -433,39 +428,26 @@ k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, 
   __syncthreads();
   const int total = rl[NR];
   const double dx = P.dx, pi_2dx = P.pi_2dx;
-  const int y = tid / S, z = tid % S;
-  const int yw0 = wv * RPW;                                 // first row of this wavefront
-  T acc[S];
-#pragma unroll
-  for (int x = 0; x < S; ++x) acc[x] = (T)0;
   for (int base = 0; base < total; base += CH) {
-    // test and compact: overlapping, growing candidates enter the list
+    // test and compact: growing candidates whose stencil meets the brick
 #pragma unroll 1
     for (int k = 0; k < CH / NT; ++k) {
       const int f = base + k * NT + tid;
       bool take = false;
-      int org = 0;
-      double4 d = make_double4(0.0, 0.0, 0.0, 0.0);
+      int t = 0;
       if (f < total) {
         int a = 0, b = NR;                                  // rl[a] <= f < rl[b]
         while (b - a > 1) {
           const int mid = (a + b) >> 1;
           if (rl[mid] <= f) a = mid; else b = mid;
         }
-        d = dep[rb[a] + (f - rl[a])];
+        t = rb[a] + (f - rl[a]);
+        const double4 d = dep[t];
         if (d.w > 0.0) {
-          int rr[3];
-          const double xb[3] = {d.x, d.y, d.z};
-          const int cb[3] = {c0x, c0y, c0z};
-          take = true;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            int r = (((int)floor(xb[c] / dx) - 1 - cb[c]) % N + N) % N;
-            if (r > N - 4) r -= N;                          // origin below the brick: -3..-1
-            take = take && r < S;
-            rr[c] = r;
-          }
-          org = packOrg(rr[0], rr[1], rr[2]);
+          const int rx = (((int)floor(d.x / dx) - 1 - c0x) % N + N) % N;
+          const int ry = (((int)floor(d.y / dx) - 1 - c0y) % N + N) % N;
+          const int rz = (((int)floor(d.z / dx) - 1 - c0z) % N + N) % N;
+          take = (rx < S || rx > N - 4) && (ry < S || ry > N - 4) && (rz < S || rz > N - 4);
         }
       }
       const unsigned long long m = __ballot(take);
@@ -473,56 +455,44 @@ k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, 
       int wbase = 0;
       if (lane == 0 && nw) wbase = atomicAdd(&cnt, nw);
       wbase = __shfl(wbase, 0, 64);
-      if (take) {
-        const int slot = wbase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        double sx, cx, sy, cy, sz, cz;
-        const int i0 = (int)floor(d.x / dx) - 1, j0 = (int)floor(d.y / dx) - 1, k0 = (int)floor(d.z / dx) - 1;
-        sincos((d.x - (double)i0 * dx) * pi_2dx, &sx, &cx);
-        sincos((d.y - (double)j0 * dx) * pi_2dx, &sy, &cy);
-        sincos((d.z - (double)k0 * dx) * pi_2dx, &sz, &cz);
-        const double W = d.w * P.inv_vol * (1.0 / 64.0);  // the three factors 1/4 of the weights
-        lorg[slot] = org;
-        lwx[slot] = V4<T>{(T)(W * (1.0 + cx)), (T)(W * (1.0 + sx)), (T)(W * (1.0 - cx)), (T)(W * (1.0 - sx))};
-        lyz[slot] = V4<T>{(T)sy, (T)cy, (T)sz, (T)cz};
-      }
+      if (take)
+        list[wbase + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = t;
     }
     __syncthreads();
     const int n = cnt;
-    for (int i = 0; i < n; ++i) {
-      const int org = __builtin_amdgcn_readfirstlane(lorg[i]);
-      const int ry = ((org >> 8) & 255) - 3;
-      if (ry + 3 < yw0 || ry >= yw0 + RPW) continue;        // misses this wavefront's rows
-      const int rx = (org & 255) - 3, rz = ((org >> 16) & 255) - 3;
-      const int dy = y - ry, dz = z - rz;
-      const V4<T> q = lyz[i];
-      // 4 w_y(dy) * 4 w_z(dz) with 4 w(a) = 1 + {cos, sin, -cos, -sin}; the
-      // x weights carry W / dx^3 and the three factors 1/4
-      T ty = (dy & 1) ? q.x : q.y;
-      ty = (dy & 2) ? -ty : ty;
-      T tz = (dz & 1) ? q.z : q.w;
-      tz = (dz & 2) ? -tz : tz;
-      const bool hit = (unsigned)dy < 4u && (unsigned)dz < 4u;
-      const T v = hit ? ((T)1 + ty) * ((T)1 + tz) : (T)0;
-      const V4<T> wx = lwx[i];
-      const T wxa[4] = {wx.x, wx.y, wx.z, wx.w};
-      // rx is uniform: one scalar branch picks the (at most 4) columns
-      SFor<-3, S>::run([&](auto xc) {
-        constexpr int X = decltype(xc)::value;
-        if (rx == X) {
-          SFor<0, 4>::run([&](auto ac) {
-            constexpr int A = decltype(ac)::value;
-            if constexpr (X + A >= 0 && X + A < S) acc[X + A] += v * wxa[A];
-          });
+    for (int i = tid; i < n; i += NT) {
+      const double4 d = dep[list[i]];
+      double wx[4], wy[4], wz[4];
+      const int i0 = cosW4(d.x, dx, pi_2dx, wx), j0 = cosW4(d.y, dx, pi_2dx, wy), k0 = cosW4(d.z, dx, pi_2dx, wz);
+      const int rx = ((i0 - c0x) % N + N) % N, ry = ((j0 - c0y) % N + N) % N, rz = ((k0 - c0z) % N + N) % N;
+      const double W = d.w * P.inv_vol;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int nx = (rx + a) % N;
+        if (nx >= S) continue;
+        const double wa = W * wx[a];
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+          const int ny = (ry + bb) % N;
+          if (ny >= S) continue;
+          const double wab = wa * wy[bb];
+#pragma unroll
+          for (int cc = 0; cc < 4; ++cc) {
+            const int nz = (rz + cc) % N;
+            if (nz >= S) continue;
+            atomicAdd(&tile[(nx * S + ny) * S + nz], wab * wz[cc]);
+          }
         }
-      });
+      }
     }
     __syncthreads();
     if (tid == 0) cnt = 0;
     __syncthreads();
   }
-#pragma unroll
-  for (int x = 0; x < S; ++x) fc[((size_t)(c0x + x) * N + (c0y + y)) * N + (c0z + z)] = acc[x];
+  for (int i = tid; i < S * S * S; i += NT) {
+    const int nx = i / (S * S), ny = (i / S) % S, nz = i % S;
+    fc[((size_t)(c0x + nx) * N + (c0y + ny)) * N + (c0z + nz)] = (T)tile[i];
+  }
 }
 
 // Fallback deposit (global atomics) of the slots that moved more than one
