@@ -97,6 +97,7 @@ struct Engine final : EngineBase {
   PT* v0b = nullptr;
   double4* dep = nullptr;            // per-slot stage position and W for the brick deposit
   int *flist = nullptr, *fcount = nullptr;
+  unsigned long long* wmaxp = nullptr;   // largest W of a droplet stage (fixed-point deposit bound)
   bool sort_enabled = true;
   // y and z passes of a chain run slab by slab (x-planes) so the y<->z
   // intermediate of one slab stays in the Infinity Cache; 1 = whole field.
@@ -185,7 +186,7 @@ struct Engine final : EngineBase {
       LD_TRY(alloc(rid, cap)); LD_TRY(alloc(rid2, cap)); LD_TRY(alloc(pkey, cap)); LD_TRY(alloc(prank, cap));
       LD_TRY(alloc(hist, NB)); LD_TRY(alloc(bsum, NBS));
       LD_TRY(alloc(x0b, 3 * cap)); LD_TRY(alloc(v0b, 4 * cap));
-      LD_TRY(alloc(dep, cap)); LD_TRY(alloc(flist, cap)); LD_TRY(alloc(fcount, 1));
+      LD_TRY(alloc(dep, cap)); LD_TRY(alloc(flist, cap)); LD_TRY(alloc(fcount, 1)); LD_TRY(alloc(wmaxp, 1));
     }
     static_assert(NBS <= 4096, "bin-start scan supports at most 4096 partial sums");
     dp.L = p.L;
@@ -412,11 +413,13 @@ struct Engine final : EngineBase {
     if (!sorted_ok || np == 0) LD_TRY(fill<T>(st2, fc, RF, (T)0));
     if (np > 0) {
       LD_TRY(fill<int>(st2, fcount, 1, 0));
+      LD_TRY(fill<unsigned long long>(st2, wmaxp, 1, 0ull));
       PStage<T, PT> ps;
       ps.np = (int)np; ps.cap = (int)cap; ps.stage = s;
       ps.x0 = x0; ps.v0 = v0; ps.sx = sx; ps.sv = sv; ps.kx = kxp; ps.kv = kvp;
       ps.gu = gup; ps.gv = gv; ps.dep = dep; ps.flist = flist; ps.fcount = fcount;
       ps.all_fallback = sorted_ok ? 0 : 1;
+      ps.wmax = wmaxp;
       ps.ah = a_coef[s] * h; ps.wh = wh; ps.P = dp;
       // 4 lanes per droplet; block count a multiple of 8 for the XCD mapping
       const unsigned nb = (unsigned)(((4 * np + 255) / 256 + 7) / 8 * 8);
@@ -426,7 +429,7 @@ struct Engine final : EngineBase {
       LD_TRY(prof_mark(LD_K_DROPLETS, st2));
       if (sorted_ok) {
         hipLaunchKernelGGL((k_brick_deposit<T, N>), dim3(Brick<N>::NB * Brick<N>::NB * Brick<N>::NB), dim3(Brick<N>::NT), 0,
-                           st2, (const double4*)dep, (const int*)hist, (int)np, fc, dp);
+                           st2, (const double4*)dep, (const int*)hist, (int)np, (const unsigned long long*)wmaxp, fc, dp);
         LD_LAUNCHED();
       }
       const unsigned fb = sorted_ok ? 64u : (unsigned)((np + 255) / 256);

@@ -193,6 +193,7 @@ template <typename T, typename PT> struct PStage {
   double4* dep;                // out: wrapped stage position and W per slot (brick deposit input)
   int* flist; int* fcount;     // out: slots whose stage bin moved > 1 bin since the sort
   int all_fallback;            // 1: slots are not bin-sorted, every slot takes the fallback deposit
+  unsigned long long* wmax;    // out: largest W > 0 of the launch (bits of a positive double); zeroed before
   double ah, wh;               // a_s * h, w_s * h
   DropPhys P;
 };
@@ -284,10 +285,22 @@ k_particles_stage(PStage<T, PT> A) {
     v += __shfl_xor(v, 2, 64);
     I[k] = v;
   }
-  if (!live) return;
-  double rhs[7], W;
-  dropletRhs(A.P, vp, m, I, rhs, W);
-  if (c != 0) return;
+  double rhs[7], W = 0.0;
+  if (live) dropletRhs(A.P, vp, m, I, rhs, W);
+  // largest growth rate of the launch: the bound of the brick deposit's fixed-point scale
+  {
+    __shared__ double wm[4];
+    double v = (live && W > 0.0) ? W : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double b = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+      if (b > 0.0) atomicMax(A.wmax, (unsigned long long)__double_as_longlong(b));
+    }
+  }
+  if (!live || c != 0) return;
   // deposit input for the brick pass (k_brick_deposit); slots whose stage
   // bin left the +-1-bin window around their sorted bin go to the fallback list
   {
@@ -330,9 +343,14 @@ k_particles_stage(PStage<T, PT> A) {
 // ------------------------------------------------------------ brick deposit
 // Deposit of W with the cosine kernel / dx^3 (particles.py:23-35, 529-575),
 // organised by output: one workgroup per brick of S^3 grid nodes accumulates
-// in an FP64 LDS tile (ds_add_f64 runs several times faster than ds_add_f32
-// on gfx950, and the sum keeps FP64 precision) and writes each node of the
-// brick exactly once.
+// in a 64-bit fixed-point LDS tile and writes each node of the brick exactly
+// once.  Integer sums do not depend on the order of the additions, so the
+// deposit is bitwise reproducible, and ds_add_u64 is the fastest LDS atomic
+// wide enough (gfx950: u64 15, f64 27, f32 196 cycles per wavefront).  The
+// scale 2^e is the largest power of two with  total * Wmax / dx^3 / 8 * 2^e
+// <= 2^62  (total = candidates of the brick, Wmax = largest W of the launch,
+// 1/8 = largest product of three weights), so no sum can overflow; one unit
+// is ~2^-62 of that bound.
 //
 // The droplets that can reach the brick are those whose stage cell lies in
 // [c0 - 2, c0 + S] per axis; their sorted bins (bins of the position at the
@@ -359,14 +377,14 @@ template <int N> struct Brick {
 
 template <typename T, int N>
 __global__ void __launch_bounds__(Brick<N>::NT)
-k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, int np, T* __restrict__ fc,
-                DropPhys P) {
+k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, int np,
+                const unsigned long long* __restrict__ wmax, T* __restrict__ fc, DropPhys P) {
   using Bk = Brick<N>;
   constexpr int S = Bk::S, NBr = Bk::NB, NBX = Bk::NBX, NR = Bk::NR, NT = Bk::NT, NZS = Bk::NZS, CH = Bk::CH;
   constexpr int TS = Bins<N>::TS;
   constexpr int BS = Bins<N>::BS, B = Bins<N>::B;
   constexpr int NBIN = B * B * B;
-  __shared__ double tile[S * S * S];
+  __shared__ unsigned long long tile[S * S * S];
   __shared__ int rb[NR], rn[NR], rl[NR + 1];
   __shared__ int list[CH];
   __shared__ int cnt;
@@ -375,7 +393,7 @@ k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, 
   const int nb = gridDim.x, per = nb / 8;
   const int br = (blockIdx.x % 8) * per + blockIdx.x / 8;
   const int c0x = (br / (NBr * NBr)) * S, c0y = ((br / NBr) % NBr) * S, c0z = (br % NBr) * S;
-  for (int i = tid; i < S * S * S; i += NT) tile[i] = 0.0;
+  for (int i = tid; i < S * S * S; i += NT) tile[i] = 0ull;
   if (tid == 0) cnt = 0;
   // slot ranges of the window's z-runs
   const int lo = (NBX == B) ? 0 : 1;
@@ -428,7 +446,11 @@ k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, 
   __syncthreads();
   const int total = rl[NR];
   const double dx = P.dx, pi_2dx = P.pi_2dx;
-  for (int base = 0; base < total; base += CH) {
+  const double bound = (double)total * __longlong_as_double((long long)*wmax) * P.inv_vol * 0.125;
+  const bool finite = isfinite(bound);
+  const int e = (bound > 0.0 && finite) ? 61 - ilogb(bound) : 0;
+  const double scale = ldexp(1.0, e), unit = ldexp(1.0, -e);
+  for (int base = 0; base < total && finite; base += CH) {
     // test and compact: growing candidates whose stencil meets the brick
 #pragma unroll 1
     for (int k = 0; k < CH / NT; ++k) {
@@ -480,7 +502,7 @@ k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, 
           for (int cc = 0; cc < 4; ++cc) {
             const int nz = (rz + cc) % N;
             if (nz >= S) continue;
-            atomicAdd(&tile[(nx * S + ny) * S + nz], wab * wz[cc]);
+            atomicAdd(&tile[(nx * S + ny) * S + nz], (unsigned long long)llrint(wab * wz[cc] * scale));
           }
         }
       }
@@ -491,7 +513,7 @@ k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, 
   }
   for (int i = tid; i < S * S * S; i += NT) {
     const int nx = i / (S * S), ny = (i / S) % S, nz = i % S;
-    fc[((size_t)(c0x + nx) * N + (c0y + ny)) * N + (c0z + nz)] = (T)tile[i];
+    fc[((size_t)(c0x + nx) * N + (c0y + ny)) * N + (c0z + nz)] = finite ? (T)((double)tile[i] * unit) : (T)NAN;
   }
 }
 

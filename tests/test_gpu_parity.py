@@ -335,6 +335,33 @@ def test_diagnostics(gpu, prec):
     d.close()
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_advance_bitwise_reproducible(gpu, prec):
+    """Two runs of the fused step from the same state give identical bits
+    (fixed-point deposit: the integer sums do not depend on atomic order)."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = preset("c1").replace(precision=prec, n_particles=20000)
+    rng = np.random.default_rng(7)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, pid = ic.initial_droplets(cfg, rng)
+    outs = []
+    for _ in range(2):
+        d = ForcedDNS(cfg)
+        ukg, ng, cg = _t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64)
+        d.set_state(ukg, ng, cg)
+        d.advance(dt, 5)
+        d.get_state(ukg, ng, cg)
+        torch.cuda.synchronize()
+        outs.append((ukg.cpu().numpy().copy(), ng.cpu().numpy().copy(), cg.cpu().numpy().copy()))
+        d.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
 def test_c1_200_steps_against_oracle_golden(gpu):
     """The 64^3 FP64 workload (10^4 droplets, 200 steps) against the committed
     CPU-oracle run (tests/golden/make_c1_golden.py): 1e-10 relative on the
