@@ -62,7 +62,15 @@ typedef struct ld_params {
   double tau_eta;        /* Kolmogorov time used by the droplets (tf)         particles.py:56 */
   double factor;         /* m/M0 = St^1.5 * factor                            particles.py:74 */
   double growthfactor;   /* 9 pi nu / (2 rho_p)                               particles.py:75 */
-  int64_t max_particles; /* droplet capacity of the context */
+  int64_t max_particles; /* droplet capacity of the context (per realization) */
+  int32_t batch;         /* realizations of an ensemble (0 or 1: one).  With batch > 1
+                            only the resident-state entry points (ld_state_*, ld_advance,
+                            ld_status*) accept the context; no counterpart in the
+                            reference, which runs one realization per job */
+  int32_t use_graph;     /* 1: ld_advance captures one step into a HIP graph (per step
+                            size and droplet count) and replays it; same results */
+  int32_t autotune;      /* 1: ld_create times the launch-geometry variants of the y
+                            passes and keeps the fastest (ld_tuning); same results */
 } ld_params;
 
 typedef struct ld_ctx ld_ctx;
@@ -106,7 +114,12 @@ int ld_rk4(ld_ctx* ctx, double h, const void* uk, void* n, const double* coord, 
  * Returns LD_ERR_BLOWUP if any step's max Re(uk) exceeded 100 N^3. */
 int ld_step(ld_ctx* ctx, double h, void* uk, void* n, double* coord, int64_t np, int32_t nsteps);
 
-/* Context-resident state (the fast path: no layout conversion per step). */
+/* Context-resident state (the fast path: no layout conversion per step).
+ * With batch B > 1 every array holds B realizations back to back: uk
+ * (B, 3, N, N, N/2+1), n (B, N, N, N), coord (B, np, 7) with np rows per
+ * realization; ld_advance steps all of them with one launch per kernel
+ * (blockIdx.z = realization).  Each realization evolves exactly as it would
+ * alone (same kernels, same arithmetic). */
 int ld_state_set(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np);
 int ld_state_get(ld_ctx* ctx, void* uk, void* n, double* coord);
 int ld_advance(ld_ctx* ctx, double h, int32_t nsteps);
@@ -115,8 +128,17 @@ int ld_advance(ld_ctx* ctx, double h, int32_t nsteps);
  * RK4/full_RHS and the loop body (forced-dns-sm.py:287-362, 582-620); droplet
  * rows are processed in input order.  The baseline of the fused path. */
 int ld_advance_plain(ld_ctx* ctx, double h, int32_t nsteps);
-/* Last step's max Re(uk) (host double) and sticky blow-up flag. */
+/* Last step's max Re(uk) (host double) and sticky blow-up flag; over an
+ * ensemble the largest maximum and whether any realization blew up. */
 int ld_status(ld_ctx* ctx, double* last_max, int32_t* blown_up);
+/* The same per realization: last_max and blown_up are host arrays of batch
+ * entries.  No counterpart in the reference (one realization per job). */
+int ld_status_all(ld_ctx* ctx, double* last_max, int32_t* blown_up);
+/* The autotuner's choice: lines per block of the y passes (*ylines, host) and
+ * the measured median times in microseconds of the two candidates (us[2],
+ * host: M and M/2 lines; zeros when the context was created without
+ * autotune).  No counterpart in the reference. */
+int ld_tuning(ld_ctx* ctx, int32_t* ylines, double* us);
 
 /* ---------------------------------------------------------- measurement */
 /* Kernel classes that ld_profile can time (HIP events recorded on the stream

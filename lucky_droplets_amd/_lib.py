@@ -40,6 +40,9 @@ class LdParams(ctypes.Structure):
         ("factor", ctypes.c_double),
         ("growthfactor", ctypes.c_double),
         ("max_particles", ctypes.c_int64),
+        ("batch", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
     ]
 
 
@@ -87,6 +90,8 @@ _SIGS = {
                          ctypes.POINTER(ctypes.c_int64)],
     "ld_coalesce": [_VP, _VP, ctypes.c_int64, _VP, ctypes.c_int64, _VP, ctypes.POINTER(ctypes.c_int64)],
     "ld_pair_stats": [_VP, _VP, ctypes.c_int64, ctypes.c_double, ctypes.c_int32, _VP, _VP, _VP],
+    "ld_status_all": [_VP, _VP, _VP],
+    "ld_tuning": [_VP, _VP, _VP],
 }
 EXPORTS = tuple(_SIGS) + ("ld_version",)
 

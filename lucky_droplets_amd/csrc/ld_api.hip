@@ -37,6 +37,15 @@ using ld::set_error;
     }                                                   \
   } while (0)
 
+// Entry points that work on one realization reject an ensemble context.
+#define LD_ONE(c)                                                                        \
+  do {                                                                                   \
+    if ((c)->eng->batch() != 1) {                                                        \
+      set_error("%s works on one realization; this context holds %d", __func__, (c)->eng->batch()); \
+      return LD_ERR_ARG;                                                                 \
+    }                                                                                    \
+  } while (0)
+
 static ld::EngineBase* make_engine(const ld_params& p) {
   using namespace ld;
 #define LD_CASE(TN, NN) \
@@ -106,30 +115,30 @@ int ld_set_stream(ld_ctx* ctx, void* s) {
 }
 
 int ld_rfft(ld_ctx* ctx, const void* u, void* fu, int nfields) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(u); LD_NONNULL(fu);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(u); LD_NONNULL(fu);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->rfft(u, fu, nfields);
 }
 
 int ld_irfft(ld_ctx* ctx, const void* fu, void* u, int nfields) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(fu); LD_NONNULL(u);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(fu); LD_NONNULL(u);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->irfft(fu, u, nfields);
 }
 
 int ld_forcing(ld_ctx* ctx, const void* uk, void* fk, double* ek_out) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(fk);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(fk);
   return ctx->eng->forcing(uk, fk, ek_out);
 }
 
 int ld_clip_zero(ld_ctx* ctx, void* x, void* out) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(x); LD_NONNULL(out);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(x); LD_NONNULL(out);
   return ctx->eng->clip_zero(x, out);
 }
 
 int ld_full_rhs(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np, void* ku, void* kn,
                 double* kp, void* fc, double visc, double forc, void* u_out, void* v_out) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n); LD_NONNULL(ku); LD_NONNULL(kn); LD_NONNULL(fc);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(n); LD_NONNULL(ku); LD_NONNULL(kn); LD_NONNULL(fc);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(kp); }
   return ctx->eng->full_rhs(uk, n, coord, np, ku, kn, kp, fc, visc, forc, u_out, v_out);
@@ -137,7 +146,7 @@ int ld_full_rhs(ld_ctx* ctx, const void* uk, const void* n, const double* coord,
 
 int ld_rk4(ld_ctx* ctx, double h, const void* uk, void* n, const double* coord, int64_t np, void* uknew, void* nnew,
            double* coord_new) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n); LD_NONNULL(uknew); LD_NONNULL(nnew);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(n); LD_NONNULL(uknew); LD_NONNULL(nnew);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(coord_new); }
   return ctx->eng->rk4(h, uk, n, coord, np, uknew, nnew, coord_new);
@@ -162,13 +171,13 @@ int ld_advance(ld_ctx* ctx, double h, int32_t nsteps) {
 }
 
 int ld_advance_plain(ld_ctx* ctx, double h, int32_t nsteps) {
-  LD_CHECK_CTX(ctx);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx);
   if (nsteps < 0) { set_error("nsteps must be >= 0"); return LD_ERR_ARG; }
   return ctx->eng->advance_plain(h, nsteps);
 }
 
 int ld_step(ld_ctx* ctx, double h, void* uk, void* n, double* coord, int64_t np, int32_t nsteps) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(n);
   if (np < 0 || nsteps < 0) { set_error("np and nsteps must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) LD_NONNULL(coord);
   int rc = ctx->eng->state_set(uk, n, coord, np);
@@ -185,6 +194,16 @@ int ld_status(ld_ctx* ctx, double* last_max, int32_t* blown_up) {
   return ctx->eng->status(last_max, blown_up);
 }
 
+int ld_tuning(ld_ctx* ctx, int32_t* ylines, double* us) {
+  LD_CHECK_CTX(ctx);
+  return ctx->eng->tuning(ylines, us);
+}
+
+int ld_status_all(ld_ctx* ctx, double* last_max, int32_t* blown_up) {
+  LD_CHECK_CTX(ctx);
+  return ctx->eng->status_all(last_max, blown_up);
+}
+
 int ld_profile(ld_ctx* ctx, int32_t kernel, int32_t enable) {
   LD_CHECK_CTX(ctx);
   if (kernel < 0 || kernel > LD_K_ZKN) { set_error("unknown kernel class %d", kernel); return LD_ERR_ARG; }
@@ -197,25 +216,25 @@ int ld_profile_read(ld_ctx* ctx, double* total_ms, int64_t* launches) {
 }
 
 int ld_spectrum(ld_ctx* ctx, const void* f, int32_t nfields, double weight, double* out) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(f); LD_NONNULL(out);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(f); LD_NONNULL(out);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->spectrum(f, nfields, weight, out);
 }
 
 int ld_flux(ld_ctx* ctx, const void* uk, const void* ku, double* out) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(ku); LD_NONNULL(out);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(ku); LD_NONNULL(out);
   return ctx->eng->flux(uk, ku, out);
 }
 
 int ld_field_stats(ld_ctx* ctx, const void* x, int32_t nfields, double* out) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(x); LD_NONNULL(out);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(x); LD_NONNULL(out);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->stats(x, nfields, out);
 }
 
 int ld_collide_pairs(ld_ctx* ctx, const double* rows, const double* radius, int64_t np, double rmax,
                      int64_t* pairs, int64_t max_pairs, int64_t* npairs) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(npairs);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(npairs);
   if (np < 0 || max_pairs < 0) { set_error("np and max_pairs must be >= 0"); return LD_ERR_ARG; }
   if (np > 1) { LD_NONNULL(rows); LD_NONNULL(radius); LD_NONNULL(pairs); }
   return ctx->eng->collide(rows, radius, np, rmax, pairs, max_pairs, npairs);
@@ -223,7 +242,7 @@ int ld_collide_pairs(ld_ctx* ctx, const double* rows, const double* radius, int6
 
 int ld_pair_stats(ld_ctx* ctx, const double* rows, int64_t np, double rmax, int32_t nbins, int64_t* counts,
                   double* wsum, double* wneg) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(counts); LD_NONNULL(wsum); LD_NONNULL(wneg);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(counts); LD_NONNULL(wsum); LD_NONNULL(wneg);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 1) LD_NONNULL(rows);
   return ctx->eng->pair_stats(rows, np, rmax, nbins, counts, wsum, wneg);
@@ -231,14 +250,14 @@ int ld_pair_stats(ld_ctx* ctx, const double* rows, int64_t np, double rmax, int3
 
 int ld_coalesce(ld_ctx* ctx, double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
                 int64_t* nevents) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(nevents);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(nevents);
   if (np < 0 || npairs < 0) { set_error("np and npairs must be >= 0"); return LD_ERR_ARG; }
   if (npairs > 0) { LD_NONNULL(rows); LD_NONNULL(pairs); LD_NONNULL(events); }
   return ctx->eng->coalesce(rows, np, pairs, npairs, events, nevents);
 }
 
 int ld_interp_cosine(ld_ctx* ctx, const double* coord, int64_t np, const void* fields, int32_t ncomp, double* out) {
-  LD_CHECK_CTX(ctx);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx);
   if (np < 0 || ncomp < 1) { set_error("np must be >= 0 and ncomp >= 1"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(fields); LD_NONNULL(out); }
   if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
@@ -246,7 +265,7 @@ int ld_interp_cosine(ld_ctx* ctx, const double* coord, int64_t np, const void* f
 }
 
 int ld_exterp_cosine_scalar(ld_ctx* ctx, const double* coord, int64_t np, const double* w, void* c) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(c);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(c);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(w); }
   if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
@@ -255,7 +274,7 @@ int ld_exterp_cosine_scalar(ld_ctx* ctx, const double* coord, int64_t np, const 
 
 int ld_prhs(ld_ctx* ctx, const double* coord, int64_t np, const void* u, const void* us, const void* n, void* fc,
             double* rhs, double* interpmat, double* exterpmat) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(u); LD_NONNULL(us); LD_NONNULL(n); LD_NONNULL(fc);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(u); LD_NONNULL(us); LD_NONNULL(n); LD_NONNULL(fc);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(rhs); LD_NONNULL(interpmat); LD_NONNULL(exterpmat); }
   if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }

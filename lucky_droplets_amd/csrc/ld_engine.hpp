@@ -6,6 +6,7 @@
 #include "ld_plain.hpp"
 #include "ld_collide.hpp"
 #include "ld_diag.hpp"
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <utility>
@@ -87,6 +88,14 @@ struct Engine final : EngineBase {
   int *flag = nullptr, *shells = nullptr;
   int smax = 0;
   int64_t cap = 0, np = 0;
+  // Ensemble of B realizations: every buffer allocated through arena() has
+  // one copy per realization, rs bytes apart (a per-realization slab), and the
+  // step's launches put the realization in blockIdx.z (rsel() in the kernels).
+  int B = 1;
+  size_t rs = 0;
+  struct ArenaSlot { void** ptr; size_t off; };
+  std::vector<ArenaSlot> arena_plan;
+  size_t arena_bytes = 0;
   double *x0 = nullptr, *sx = nullptr, *kxp = nullptr;
   PT *v0 = nullptr, *sv = nullptr, *kvp = nullptr;
   double *rows_a = nullptr, *rows_b = nullptr, *wrow = nullptr;
@@ -115,8 +124,13 @@ struct Engine final : EngineBase {
   explicit Engine(const ld_params& prm) : p(prm) {}
   ~Engine() override {
     if (st2) { (void)hipStreamSynchronize(st2); (void)hipStreamDestroy(st2); }
+    if (stg) { (void)hipStreamSynchronize(stg); (void)hipStreamDestroy(stg); }
+    for (StepGraph& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     if (evA) (void)hipEventDestroy(evA);
+    if (evG0) (void)hipEventDestroy(evG0);
+    if (evG1) (void)hipEventDestroy(evG1);
     if (evB) (void)hipEventDestroy(evB);
     for (void* a : allocs) (void)hipFree(a);
   }
@@ -127,6 +141,62 @@ struct Engine final : EngineBase {
     LD_HIP(hipMalloc(&q, count * sizeof(X)));
     allocs.push_back(q);
     ptr = (X*)q;
+    return LD_OK;
+  }
+  // Reserve a per-realization buffer in the slab; arena_commit() allocates
+  // B slabs and sets every reserved pointer to realization 0's copy.
+  template <class X> void arena(X*& ptr, size_t count) {
+    if (count == 0) count = 1;
+    arena_bytes = (arena_bytes + 255) / 256 * 256;
+    arena_plan.push_back({reinterpret_cast<void**>(&ptr), arena_bytes});
+    arena_bytes += count * sizeof(X);
+  }
+  int arena_commit() {
+    rs = (arena_bytes + 255) / 256 * 256;
+    void* base = nullptr;
+    LD_HIP(hipMalloc(&base, rs * (size_t)B));
+    allocs.push_back(base);
+    for (const ArenaSlot& a : arena_plan) *a.ptr = static_cast<char*>(base) + a.off;
+    return LD_OK;
+  }
+  // realization b's copy of an arena buffer (host-side address arithmetic)
+  template <class X> X* at(X* ptr, int b) const {
+    return reinterpret_cast<X*>(reinterpret_cast<char*>(ptr) + (size_t)b * rs);
+  }
+  int batch() const override { return B; }
+
+  // Autotuner: time the 6-field y-inverse pass with M and M/2 lines per block
+  // (after one warm-up launch each, the median of five) and keep the faster.
+  double tune_us[2] = {0.0, 0.0};
+  int autotune() {
+    LD_TRY(fill<C<T>>(st, w6, 6 * CF, C<T>{}));
+    hipEvent_t e0, e1;
+    LD_HIP(hipEventCreate(&e0));
+    LD_HIP(hipEventCreate(&e1));
+    const int cand[2] = {M, MH};
+    for (int c = 0; c < 2; ++c) {
+      my = cand[c];
+      std::vector<float> t;
+      for (int r = 0; r < 6; ++r) {
+        LD_HIP(hipEventRecord(e0, st));
+        LD_TRY((pass<1, +1>(st, w6, w6, 6, (T)1)));
+        LD_HIP(hipEventRecord(e1, st));
+        LD_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        LD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0) t.push_back(ms);
+      }
+      std::sort(t.begin(), t.end());
+      tune_us[c] = 1000.0 * t[t.size() / 2];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    my = tune_us[1] < tune_us[0] ? MH : M;
+    return LD_OK;
+  }
+  int tuning(int32_t* ylines, double* us) override {
+    if (ylines) *ylines = my;
+    if (us) { us[0] = tune_us[0]; us[1] = tune_us[1]; }
     return LD_OK;
   }
 
@@ -142,6 +212,8 @@ struct Engine final : EngineBase {
     }
     if (p.max_particles < 0 || p.max_particles > (int64_t)0x7fffffff) { set_error("max_particles out of range"); return LD_ERR_ARG; }
     cap = p.max_particles;
+    if (p.batch < 0 || p.batch > 4096) { set_error("batch must be in [0, 4096]"); return LD_ERR_ARG; }
+    B = p.batch > 1 ? p.batch : 1;
     // LD_SERIAL=1: the flow stream waits for each stage's density chain, so the
     // two chains never run concurrently (per-kernel timing without overlap)
     if (const char* e = getenv("LD_SERIAL")) serial = atoi(e) != 0;
@@ -158,35 +230,44 @@ struct Engine final : EngineBase {
     LD_TRY(alloc(tw, N));
     LD_HIP(hipMemcpy(tw, h.data(), N * sizeof(C<T>), hipMemcpyHostToDevice));
     LD_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));   // default priority
+    use_graph = p.use_graph != 0;
+    if (use_graph) {
+      LD_HIP(hipStreamCreateWithFlags(&stg, hipStreamNonBlocking));
+      LD_HIP(hipEventCreateWithFlags(&evG0, hipEventDisableTiming));
+      LD_HIP(hipEventCreateWithFlags(&evG1, hipEventDisableTiming));
+    }
     LD_HIP(hipEventCreateWithFlags(&evA, hipEventDisableTiming));
     LD_HIP(hipEventCreateWithFlags(&evB, hipEventDisableTiming));
-    LD_TRY(alloc(uk, 3 * CF)); LD_TRY(alloc(us, 3 * CF)); LD_TRY(alloc(acc, 3 * CF)); LD_TRY(alloc(w6, 6 * CF));
-    LD_TRY(alloc(wn, 4 * CF));
-    LD_TRY(alloc(ubuf[0], 3 * RF)); LD_TRY(alloc(ubuf[1], 3 * RF)); LD_TRY(alloc(v, 3 * RF));
-    LD_TRY(alloc(gu[0], RF)); LD_TRY(alloc(gu[1], RF)); LD_TRY(alloc(gv, RF));
-    LD_TRY(alloc(n_state, RF)); LD_TRY(alloc(n_stage, RF)); LD_TRY(alloc(nacc, RF)); LD_TRY(alloc(fc, RF));
+    // per-realization state and work buffers (one slab per realization)
+    arena(uk, 3 * CF); arena(us, 3 * CF); arena(acc, 3 * CF); arena(w6, 6 * CF); arena(wn, 4 * CF);
+    arena(ubuf[0], 3 * RF); arena(ubuf[1], 3 * RF); arena(v, 3 * RF);
+    arena(gu[0], RF); arena(gu[1], RF); arena(gv, RF);
+    arena(n_state, RF); arena(n_stage, RF); arena(nacc, RF); arena(fc, RF);
     const int npart = NBLK_Z1 > NBLK_SUM ? NBLK_Z1 : NBLK_SUM;
-    LD_TRY(alloc(partials, 2 * (size_t)npart));
-    LD_TRY(alloc(partials2, 2 * (size_t)npart));
-    LD_TRY(alloc(clip_stage, 4)); LD_TRY(alloc(clip_end, 4)); LD_TRY(alloc(clip_none, 4));
-    LD_TRY(alloc(maxpart, NBLK_X)); LD_TRY(alloc(ek, Nf)); LD_TRY(alloc(fac, Nf)); LD_TRY(alloc(lastmax, 1));
-    LD_TRY(alloc(flag, 1)); LD_TRY(alloc(shells, 8));
-    LD_HIP(hipMemset(fac, 0, Nf * sizeof(double)));
-    LD_HIP(hipMemset(ek, 0, Nf * sizeof(double)));
-    LD_HIP(hipMemset(flag, 0, sizeof(int)));
+    arena(partials, 2 * (size_t)npart); arena(partials2, 2 * (size_t)npart);
+    arena(clip_stage, 4); arena(clip_end, 4); arena(clip_none, 4);
+    arena(maxpart, NBLK_X); arena(ek, Nf); arena(fac, Nf); arena(lastmax, 1); arena(flag, 1);
+    if (cap > 0) {
+      arena(x0, 3 * cap); arena(sx, 3 * cap); arena(kxp, 3 * cap);
+      arena(v0, 4 * cap); arena(sv, 4 * cap); arena(kvp, 4 * cap);
+      arena(rows_a, 7 * cap); arena(rows_b, 7 * cap); arena(wrow, cap);
+      arena(rid, cap); arena(rid2, cap); arena(pkey, cap); arena(prank, cap);
+      arena(hist, NB); arena(bsum, NBS);
+      arena(x0b, 3 * cap); arena(v0b, 4 * cap);
+      arena(dep, cap); arena(flist, cap); arena(fcount, 1); arena(wmaxp, 1);
+    }
+    LD_TRY(arena_commit());
+    LD_TRY(alloc(shells, 8));
     LD_HIP(hipMemcpy(shells, p.shell_no, 8 * sizeof(int), hipMemcpyHostToDevice));
     const double cn[4] = {1.0, 1.0, 0.0, 0.0};
-    LD_HIP(hipMemcpy(clip_none, cn, sizeof(cn), hipMemcpyHostToDevice));
-    LD_HIP(hipMemcpy(clip_stage, cn, sizeof(cn), hipMemcpyHostToDevice));
-    LD_HIP(hipMemcpy(clip_end, cn, sizeof(cn), hipMemcpyHostToDevice));
-    if (cap > 0) {
-      LD_TRY(alloc(x0, 3 * cap)); LD_TRY(alloc(sx, 3 * cap)); LD_TRY(alloc(kxp, 3 * cap));
-      LD_TRY(alloc(v0, 4 * cap)); LD_TRY(alloc(sv, 4 * cap)); LD_TRY(alloc(kvp, 4 * cap));
-      LD_TRY(alloc(rows_a, 7 * cap)); LD_TRY(alloc(rows_b, 7 * cap)); LD_TRY(alloc(wrow, cap));
-      LD_TRY(alloc(rid, cap)); LD_TRY(alloc(rid2, cap)); LD_TRY(alloc(pkey, cap)); LD_TRY(alloc(prank, cap));
-      LD_TRY(alloc(hist, NB)); LD_TRY(alloc(bsum, NBS));
-      LD_TRY(alloc(x0b, 3 * cap)); LD_TRY(alloc(v0b, 4 * cap));
-      LD_TRY(alloc(dep, cap)); LD_TRY(alloc(flist, cap)); LD_TRY(alloc(fcount, 1)); LD_TRY(alloc(wmaxp, 1));
+    for (int b = 0; b < B; ++b) {
+      LD_HIP(hipMemset(at(fac, b), 0, Nf * sizeof(double)));
+      LD_HIP(hipMemset(at(ek, b), 0, Nf * sizeof(double)));
+      LD_HIP(hipMemset(at(flag, b), 0, sizeof(int)));
+      LD_HIP(hipMemset(at(lastmax, b), 0, sizeof(double)));
+      LD_HIP(hipMemcpy(at(clip_none, b), cn, sizeof(cn), hipMemcpyHostToDevice));
+      LD_HIP(hipMemcpy(at(clip_stage, b), cn, sizeof(cn), hipMemcpyHostToDevice));
+      LD_HIP(hipMemcpy(at(clip_end, b), cn, sizeof(cn), hipMemcpyHostToDevice));
     }
     static_assert(NBS <= 4096, "bin-start scan supports at most 4096 partial sums");
     dp.L = p.L;
@@ -194,6 +275,7 @@ struct Engine final : EngineBase {
     dp.pi_2dx = M_PI / (2.0 * dp.dx);
     dp.inv_vol = 1.0 / (dp.dx * dp.dx * dp.dx);
     dp.factor = p.factor; dp.growth = p.growthfactor; dp.st_s = p.st_s; dp.tau_eta = p.tau_eta; dp.g = p.g;
+    if (p.autotune) LD_TRY(autotune());
     return LD_OK;
   }
 
@@ -201,14 +283,21 @@ struct Engine final : EngineBase {
 
   // ------------------------------------------------------------ launch helpers
   // Line pass over x-planes [x0, x0 + nx) (AXIS 1) or all lines (AXIS 0).
+  // y passes run with MY lines per block: M, or M/2 when the autotuner
+  // measured that faster (the lines a block owns do not change the arithmetic).
+  static constexpr int MH = M / 2;
+  int my = M;
   template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale, int x0 = 0,
                                         int nx = N) {
     const int kid = AXIS == 1 ? LD_K_YPASS : -2;
-    const int nblk = AXIS == 1 ? nx * Nf / M : NBLK_X;
     const int qoff = AXIS == 1 ? x0 * Nf : 0;
     LD_TRY(prof_mark(kid, s));
-    hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(nblk, nf), dim3(M * N1), 0, s, in, out, CF,
-                       (const C<T>*)tw, scale, qoff);
+    if (AXIS == 1 && my == MH)
+      hipLaunchKernelGGL((k_pass<T, N, MH, AXIS, DIR>), dim3(nx * Nf / MH, nf, B), dim3(MH * N1), 0, s, in, out, CF,
+                         (const C<T>*)tw, scale, qoff, rs);
+    else
+      hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(AXIS == 1 ? nx * Nf / M : NBLK_X, nf, B), dim3(M * N1), 0, s,
+                         in, out, CF, (const C<T>*)tw, scale, qoff, rs);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
@@ -217,30 +306,30 @@ struct Engine final : EngineBase {
   template <int P, int FIN, int FOUT, class Op> int zrun(hipStream_t s, const Op& op, int kid = -2, int x0 = 0,
                                                          int nx = N) {
     LD_TRY(prof_mark(kid, s));
-    hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(nx * N / P), dim3(ZThreads<N, P, FIN, FOUT>::v), 0, s, op,
-                       (const C<T>*)tw, x0 * N);
+    hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(nx * N / P, 1, B), dim3(ZThreads<N, P, FIN, FOUT>::v), 0,
+                       s, op, (const C<T>*)tw, x0 * N, rs);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
   }
   template <class X> int fill(hipStream_t s, X* x, size_t n, X val) {
-    hipLaunchKernelGGL((k_fill<X>), dim3(1024), dim3(256), 0, s, x, n, val);
+    hipLaunchKernelGGL((k_fill<X>), dim3(1024, 1, B), dim3(256), 0, s, x, n, val, rs);
     LD_LAUNCHED();
     return LD_OK;
   }
   int shell(hipStream_t s, const C<T>* uin, double fscale) {
     const double norm0 = 1.0 / ((double)N * N * N * N * N * N / std::pow(2.0 * M_PI, 3.0));
     if (p.nshells == 0) return LD_OK;
-    hipLaunchKernelGGL((k_shell<T, N>), dim3(1), dim3(256), 0, s, uin, CF, smax, (const int*)shells, p.nshells,
-                       p.f0, fscale, norm0, ek, fac);
+    hipLaunchKernelGGL((k_shell<T, N>), dim3(1, 1, B), dim3(256), 0, s, uin, CF, smax, (const int*)shells, p.nshells,
+                       p.f0, fscale, norm0, ek, fac, rs);
     LD_LAUNCHED();
     return LD_OK;
   }
   int clip_sums(hipStream_t s, const T* x, double* clip) {
-    hipLaunchKernelGGL((k_sum2<T>), dim3(NBLK_SUM), dim3(256), 0, s, x, RF, partials);
+    hipLaunchKernelGGL((k_sum2<T>), dim3(NBLK_SUM, 1, B), dim3(256), 0, s, x, RF, partials, rs);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_clip_final, dim3(1), dim3(256), 0, s, (const double*)partials, NBLK_SUM,
-                       1.0 / ((double)N * N * N), clip);
+    hipLaunchKernelGGL(k_clip_final, dim3(1, 1, B), dim3(256), 0, s, (const double*)partials, NBLK_SUM,
+                       1.0 / ((double)N * N * N), clip, rs);
     LD_LAUNCHED();
     return LD_OK;
   }
@@ -324,7 +413,8 @@ struct Engine final : EngineBase {
 
   int clip_zero(void* x, void* out) override {
     LD_TRY(clip_sums(st, (const T*)x, clip_end));
-    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st, (T*)x, (T*)out, RF, (const double*)clip_end, 1);
+    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st, (T*)x, (T*)out, RF, (const double*)clip_end, 1,
+                       (size_t)0);
     LD_LAUNCHED();
     return LD_OK;
   }
@@ -334,7 +424,8 @@ struct Engine final : EngineBase {
   // u x omega in w6[0..3).
   int nonlinear(hipStream_t s, const C<T>* usp, T* uphys, V4<T>* g = nullptr) {
     LD_TRY(prof_mark(LD_K_XUW, s));
-    hipLaunchKernelGGL((k_x_uw3<T, N, M3>), dim3(NBLK_X3), dim3(3 * M3 * N1), 0, s, usp, w6, CF, (const C<T>*)tw, inv());
+    hipLaunchKernelGGL((k_x_uw3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, s, usp, w6, CF, (const C<T>*)tw,
+                       inv(), rs);
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XUW, s));
     OpZCross<T, N, P6> oc{w6, CF, uphys, g, inv()};
@@ -372,7 +463,8 @@ struct Engine final : EngineBase {
       LD_TRY((pass<1, -1>(s, wn, wn, 4, (T)1, x0, nx)));
     }
     LD_TRY(prof_mark(LD_K_XKN, s));
-    hipLaunchKernelGGL((k_x_kn4<T, N, M4>), dim3(NBLK_X4), dim3(4 * M4 * N1), 0, s, wn, CF, (const C<T>*)tw, inv());
+    hipLaunchKernelGGL((k_x_kn4<T, N, M4>), dim3(NBLK_X4, 1, B), dim3(4 * M4 * N1), 0, s, wn, CF, (const C<T>*)tw,
+                       inv(), rs);
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XKN, s));
     for (int x0 = 0; x0 < N; x0 += nx) {
@@ -404,7 +496,7 @@ struct Engine final : EngineBase {
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
     ra.wacc = (T)wh; ra.anext = (T)anext; ra.scale = inv(); ra.acc_init = (s == 0);
     LD_TRY(prof_mark(LD_K_XRHS, st));
-    hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw);
+    hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw, rs);
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XRHS, st));
     // v, nv need the previous density chain done (n_s, clip, wn and v free)
@@ -420,21 +512,25 @@ struct Engine final : EngineBase {
       ps.gu = gup; ps.gv = gv; ps.dep = dep; ps.flist = flist; ps.fcount = fcount;
       ps.all_fallback = sorted_ok ? 0 : 1;
       ps.wmax = wmaxp;
-      ps.ah = a_coef[s] * h; ps.wh = wh; ps.P = dp;
+      ps.ah = a_coef[s] * h; ps.wh = wh; ps.P = dp; ps.rs = rs;
       // 4 lanes per droplet; block count a multiple of 8 for the XCD mapping
       const unsigned nb = (unsigned)(((4 * np + 255) / 256 + 7) / 8 * 8);
       LD_TRY(prof_mark(LD_K_DROPLETS, st2));
-      hipLaunchKernelGGL((k_particles_stage<T, PT, N>), dim3(nb), dim3(256), 0, st2, ps);
+      if (B == 1)
+        hipLaunchKernelGGL((k_particles_stage<T, PT, N, false>), dim3(nb), dim3(256), 0, st2, ps);
+      else
+        hipLaunchKernelGGL((k_particles_stage<T, PT, N, true>), dim3(nb, 1, B), dim3(256), 0, st2, ps);
       LD_LAUNCHED();
       LD_TRY(prof_mark(LD_K_DROPLETS, st2));
       if (sorted_ok) {
-        hipLaunchKernelGGL((k_brick_deposit<T, N>), dim3(Brick<N>::NB * Brick<N>::NB * Brick<N>::NB), dim3(Brick<N>::NT), 0,
-                           st2, (const double4*)dep, (const int*)hist, (int)np, (const unsigned long long*)wmaxp, fc, dp);
+        hipLaunchKernelGGL((k_brick_deposit<T, N>), dim3(Brick<N>::NB * Brick<N>::NB * Brick<N>::NB, 1, B),
+                           dim3(Brick<N>::NT), 0, st2, (const double4*)dep, (const int*)hist, (int)np,
+                           (const unsigned long long*)wmaxp, fc, dp, rs);
         LD_LAUNCHED();
       }
       const unsigned fb = sorted_ok ? 64u : (unsigned)((np + 255) / 256);
-      hipLaunchKernelGGL((k_fallback_deposit<T, N>), dim3(fb), dim3(256), 0, st2, (const double4*)dep,
-                         (const int*)flist, (const int*)fcount, fc, dp);
+      hipLaunchKernelGGL((k_fallback_deposit<T, N>), dim3(fb, 1, B), dim3(256), 0, st2, (const double4*)dep,
+                         (const int*)flist, (const int*)fcount, fc, dp, rs);
       LD_LAUNCHED();
     }
     OpZKn<T, N, P1> ok;
@@ -443,8 +539,8 @@ struct Engine final : EngineBase {
     ok.wacc = (T)wh; ok.anext = (T)anext; ok.acc_init = (s == 0);
     ok.sum_target = (s < 3) ? 0 : 1; ok.partials = partials2;
     LD_TRY(density_rhs(st2, fc, &ok));
-    hipLaunchKernelGGL(k_clip_final, dim3(1), dim3(256), 0, st2, (const double*)partials2, NBLK_Z1,
-                       1.0 / ((double)N * N * N), (s < 3) ? clip_stage : clip_end);
+    hipLaunchKernelGGL(k_clip_final, dim3(1, 1, B), dim3(256), 0, st2, (const double*)partials2, NBLK_Z1,
+                       1.0 / ((double)N * N * N), (s < 3) ? clip_stage : clip_end, rs);
     LD_LAUNCHED();
     LD_HIP(hipEventRecord(evB, st2));
     if (serial) LD_TRY(join(st, st2, evB));
@@ -470,7 +566,8 @@ struct Engine final : EngineBase {
   // density stream; reality round trip, projection and blow-up test on the
   // flow stream.
   int epilogue() {
-    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st2, nacc, n_state, RF, (const double*)clip_end, 1);
+    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024, 1, B), dim3(256), 0, st2, nacc, n_state, RF, (const double*)clip_end,
+                       1, rs);
     LD_LAUNCHED();
     LD_TRY((pass<0, +1>(st, acc, w6, 3, inv())));
     OpZRound<T, N, P3> orr{w6, CF, inv()};
@@ -480,47 +577,62 @@ struct Engine final : EngineBase {
       LD_TRY((zrun<P3, 3, 3>(st, orr, -2, x0, nx)));
       LD_TRY((pass<1, -1>(st, w6, w6, 3, (T)1, x0, nx)));
     }
-    hipLaunchKernelGGL((k_x_proj<T, N, M>), dim3(NBLK_X), dim3(M * N1), 0, st, (const C<T>*)w6, uk, CF,
-                       (const C<T>*)tw, maxpart);
+    hipLaunchKernelGGL((k_x_proj<T, N, M>), dim3(NBLK_X, 1, B), dim3(M * N1), 0, st, (const C<T>*)w6, uk, CF,
+                       (const C<T>*)tw, maxpart, rs);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, st, (const double*)maxpart, NBLK_X,
-                       100.0 * (double)N * N * N, flag, lastmax);
+    hipLaunchKernelGGL(k_max_final, dim3(1, 1, B), dim3(256), 0, st, (const double*)maxpart, NBLK_X,
+                       100.0 * (double)N * N * N, flag, lastmax, rs);
     LD_LAUNCHED();
     return LD_OK;
   }
 
   // ------------------------------------------------------------ state
+  // Droplet rows of every realization: coord holds B blocks of n_ rows.
   int set_rows(const double* coord, int64_t n_) {
     sorted_ok = false;
     if (n_ > cap) { set_error("%lld droplets exceed the context capacity %lld", (long long)n_, (long long)cap); return LD_ERR_CAPACITY; }
     np = n_;
     if (np > 0) {
-      hipLaunchKernelGGL((k_rows_to_soa<PT>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, coord, (int)np,
-                         (int)cap, x0, v0);
-      LD_LAUNCHED();
-      hipLaunchKernelGGL(k_iota64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, rid, (int)np);
-      LD_LAUNCHED();
+      for (int b = 0; b < B; ++b) {
+        hipLaunchKernelGGL((k_rows_to_soa<PT>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
+                           coord + (size_t)b * np * 7, (int)np, (int)cap, at(x0, b), at(v0, b));
+        LD_LAUNCHED();
+        hipLaunchKernelGGL(k_iota64, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, at(rid, b), (int)np);
+        LD_LAUNCHED();
+      }
     }
     return LD_OK;
   }
   int get_rows(double* coord) {
     if (np > 0) {
-      hipLaunchKernelGGL((k_soa_to_rows<PT>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
-                         (const double*)x0, (const PT*)v0, (const int64_t*)rid, (int)np, (int)cap, coord);
-      LD_LAUNCHED();
+      for (int b = 0; b < B; ++b) {
+        hipLaunchKernelGGL((k_soa_to_rows<PT>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st,
+                           (const double*)at(x0, b), (const PT*)at(v0, b), (const int64_t*)at(rid, b), (int)np,
+                           (int)cap, coord + (size_t)b * np * 7);
+        LD_LAUNCHED();
+      }
     }
     return LD_OK;
   }
+  // uk, n and coord hold B realizations back to back.
   int state_set(const void* ukin, const void* nin, const double* coord, int64_t n_) override {
     LD_TRY(join(st, st2, evB));
-    LD_HIP(hipMemcpyAsync(uk, ukin, 3 * CF * sizeof(C<T>), hipMemcpyDeviceToDevice, st));
-    LD_HIP(hipMemcpyAsync(n_state, nin, RF * sizeof(T), hipMemcpyDeviceToDevice, st));
+    for (int b = 0; b < B; ++b) {
+      LD_HIP(hipMemcpyAsync(at(uk, b), (const C<T>*)ukin + (size_t)b * 3 * CF, 3 * CF * sizeof(C<T>),
+                            hipMemcpyDeviceToDevice, st));
+      LD_HIP(hipMemcpyAsync(at(n_state, b), (const T*)nin + (size_t)b * RF, RF * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
     return set_rows(coord, n_);
   }
   int state_get(void* ukout, void* nout, double* coord) override {
     LD_TRY(join(st, st2, evB));
-    if (ukout) LD_HIP(hipMemcpyAsync(ukout, uk, 3 * CF * sizeof(C<T>), hipMemcpyDeviceToDevice, st));
-    if (nout) LD_HIP(hipMemcpyAsync(nout, n_state, RF * sizeof(T), hipMemcpyDeviceToDevice, st));
+    for (int b = 0; b < B; ++b) {
+      if (ukout)
+        LD_HIP(hipMemcpyAsync((C<T>*)ukout + (size_t)b * 3 * CF, at(uk, b), 3 * CF * sizeof(C<T>),
+                              hipMemcpyDeviceToDevice, st));
+      if (nout)
+        LD_HIP(hipMemcpyAsync((T*)nout + (size_t)b * RF, at(n_state, b), RF * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
     if (coord) LD_TRY(get_rows(coord));
     return LD_OK;
   }
@@ -532,45 +644,123 @@ struct Engine final : EngineBase {
     if (np < 1 || !sort_enabled) return LD_OK;
     const unsigned pb = (unsigned)((np + 255) / 256);
     LD_TRY(fill<int>(s, hist, NB, 0));
-    hipLaunchKernelGGL((k_bin_count<N>), dim3(pb), dim3(256), 0, s, (const double*)x0, (int)np, (int)cap, p.L,
-                       dp.dx, pkey, prank, hist);
+    hipLaunchKernelGGL((k_bin_count<N>), dim3(pb, 1, B), dim3(256), 0, s, (const double*)x0, (int)np, (int)cap, p.L,
+                       dp.dx, pkey, prank, hist, rs);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_local, dim3(NBS), dim3(1024), 0, s, hist, NB, bsum);
+    hipLaunchKernelGGL(k_scan_local, dim3(NBS, 1, B), dim3(1024), 0, s, hist, NB, bsum, rs);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, s, bsum, NBS);
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1, 1, B), dim3(1024), 0, s, bsum, NBS, rs);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_add, dim3((NB + 255) / 256), dim3(256), 0, s, hist, NB, (const int*)bsum);
+    hipLaunchKernelGGL(k_scan_add, dim3((NB + 255) / 256, 1, B), dim3(256), 0, s, hist, NB, (const int*)bsum, rs);
     LD_LAUNCHED();
-    hipLaunchKernelGGL((k_bin_scatter<PT>), dim3(pb), dim3(256), 0, s, (int)np, (int)cap, (const int*)pkey,
+    hipLaunchKernelGGL((k_bin_scatter<PT>), dim3(pb, 1, B), dim3(256), 0, s, (int)np, (int)cap, (const int*)pkey,
                        (const int*)prank, (const int*)hist, (const double*)x0, (const PT*)v0, (const int64_t*)rid,
-                       x0b, v0b, rid2);
+                       x0b, v0b, rid2, rs);
     LD_LAUNCHED();
+    swap_sorted();
+    return LD_OK;
+  }
+  // The sort scatters into the second set of droplet buffers; they become current.
+  void swap_sorted() {
     std::swap(x0, x0b);
     std::swap(v0, v0b);
     std::swap(rid, rid2);
     sorted_ok = true;
+    parity ^= 1;
+  }
+  bool sorts() const { return np >= 1 && sort_enabled; }
+
+  // ------------------------------------------------------------ HIP graphs
+  // With graphs on, one step (sort, four stages, epilogue on both streams) is
+  // captured once per (h, droplet count, buffer parity) and replayed with one
+  // hipGraphLaunch.  The capture runs the host side of the step (the sort's
+  // buffer swap included) without executing it; a replay repeats that swap.
+  // Steps run on an internal stream, joined to the caller's stream by events.
+  struct StepGraph { hipGraphExec_t exec = nullptr; double h = 0.0; int64_t np = -1; };
+  bool use_graph = false;
+  hipStream_t stg = nullptr;
+  hipEvent_t evG0 = nullptr, evG1 = nullptr;   // caller <-> internal stream joins
+  StepGraph graphs[2];
+  int parity = 0;                    // droplet buffers swapped an odd number of times
+
+  int step_body(double h) {
+    LD_TRY(sort_droplets(st2));
+    LD_TRY(rk4_stages(h));
+    return epilogue();
+  }
+  int graph_step(double h) {
+    StepGraph& g = graphs[parity];
+    if (!g.exec || g.h != h || g.np != np) {
+      if (g.exec) { LD_HIP(hipGraphExecDestroy(g.exec)); g.exec = nullptr; }
+      const int par = parity;
+      LD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = join(st2, st, evA);                  // fork the density stream into the capture
+      if (rc == LD_OK) rc = step_body(h);
+      if (rc == LD_OK) rc = join(st, st2, evB);     // and join it back
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(st, &graph);
+      if (rc != LD_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != LD_OK) return rc;
+        LD_HIP(e);
+      }
+      const hipError_t ei = hipGraphInstantiate(&graphs[par].exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      LD_HIP(ei);
+      graphs[par].h = h;
+      graphs[par].np = np;
+      LD_HIP(hipGraphLaunch(graphs[par].exec, st));
+      return LD_OK;
+    }
+    LD_HIP(hipGraphLaunch(g.exec, st));
+    if (sorts()) swap_sorted();
     return LD_OK;
   }
 
   int advance(double h, int nsteps) override {
-    LD_TRY(join(st2, st, evA));      // state written on the caller's stream
-    for (int i = 0; i < nsteps; ++i) {
-      LD_TRY(sort_droplets(st2));
-      LD_TRY(rk4_stages(h));
-      LD_TRY(epilogue());
+    if (use_graph && prof_id < 0) {
+      // steps on the internal stream (a legacy default stream cannot be captured)
+      hipStream_t caller = st;
+      LD_TRY(join(stg, caller, evG0));
+      st = stg;
+      int rc = LD_OK;
+      for (int i = 0; i < nsteps && rc == LD_OK; ++i) rc = graph_step(h);
+      st = caller;
+      LD_TRY(rc);
+      LD_TRY(join(caller, stg, evG1));
+    } else {
+      LD_TRY(join(st2, st, evA));      // state written on the caller's stream
+      for (int i = 0; i < nsteps; ++i) LD_TRY(step_body(h));
+      LD_TRY(join(st, st2, evB));      // caller's stream sees the whole step
     }
-    LD_TRY(join(st, st2, evB));      // caller's stream sees the whole step
-    int32_t blown = 0;
-    LD_HIP(hipMemcpyAsync(&blown, flag, sizeof(int), hipMemcpyDeviceToHost, st));
-    LD_HIP(hipStreamSynchronize(st));
-    if (blown) { set_error("Threshold exceeded: max Re(u_hat) > 100 N^3"); return LD_ERR_BLOWUP; }
+    std::vector<double> lm(B);
+    std::vector<int32_t> bl(B);
+    LD_TRY(status_all(lm.data(), bl.data()));
+    for (int b = 0; b < B; ++b)
+      if (bl[b]) { set_error("Threshold exceeded: max Re(u_hat) > 100 N^3 (realization %d)", b); return LD_ERR_BLOWUP; }
     return LD_OK;
   }
-  int status(double* last_max, int32_t* blown) override {
+  // Largest max Re(u_hat) of the last epilogue and the blow-up flag, per
+  // realization (host arrays of B entries).
+  int status_all(double* last_max, int32_t* blown) override {
     LD_TRY(join(st, st2, evB));
-    if (last_max) LD_HIP(hipMemcpyAsync(last_max, lastmax, sizeof(double), hipMemcpyDeviceToHost, st));
-    if (blown) LD_HIP(hipMemcpyAsync(blown, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    for (int b = 0; b < B; ++b) {
+      if (last_max) LD_HIP(hipMemcpyAsync(last_max + b, at(lastmax, b), sizeof(double), hipMemcpyDeviceToHost, st));
+      if (blown) LD_HIP(hipMemcpyAsync(blown + b, at(flag, b), sizeof(int), hipMemcpyDeviceToHost, st));
+    }
     LD_HIP(hipStreamSynchronize(st));
+    return LD_OK;
+  }
+  // Over the ensemble: the largest maximum and whether any realization blew up.
+  int status(double* last_max, int32_t* blown) override {
+    std::vector<double> lm(B);
+    std::vector<int32_t> bl(B);
+    LD_TRY(status_all(lm.data(), bl.data()));
+    double m = lm[0];
+    int32_t any = 0;
+    for (int b = 0; b < B; ++b) { m = lm[b] > m ? lm[b] : m; any |= bl[b]; }
+    if (last_max) *last_max = m;
+    if (blown) *blown = any;
     return LD_OK;
   }
 
@@ -603,7 +793,7 @@ struct Engine final : EngineBase {
     ra.nl = w6; ra.us = ukp; ra.ubase = ukp; ra.acc = nullptr; ra.usnext = nullptr; ra.ku = (C<T>*)ku;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit * visc); ra.tps = (T)p.tps;
     ra.wacc = (T)0; ra.anext = (T)0; ra.scale = inv(); ra.acc_init = 0;
-    hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw);
+    hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw, rs);
     LD_LAUNCHED();
     // the physical-space v the droplets see uses visc as well (forced-dns-sm.py:319-321)
     LD_TRY(velocity(st, vphys, nullptr, (const T*)nin, clip_none, nullptr));
@@ -669,7 +859,7 @@ struct Engine final : EngineBase {
   }
   int p_clip_zero(T* x, T* out) {   // clip_zero (:278-285): sums, clip x in place, out = x*old/new
     LD_TRY(clip_sums(st, x, clip_end));
-    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st, x, out, RF, (const double*)clip_end, 1);
+    hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024), dim3(256), 0, st, x, out, RF, (const double*)clip_end, 1, (size_t)0);
     LD_LAUNCHED();
     return LD_OK;
   }
@@ -837,11 +1027,12 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL(k_coll_count, dim3(pb), dim3(256), 0, st, rows, (int)n_, p.L, cell, nc, ckey, crank, chist);
     LD_LAUNCHED();
     const int nbs = (ncells + 4095) / 4096;
-    hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(1024), 0, st, chist, ncells, cbsum);
+    hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(1024), 0, st, chist, ncells, cbsum, (size_t)0);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, st, cbsum, nbs);
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(1024), 0, st, cbsum, nbs, (size_t)0);
     LD_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_add, dim3((ncells + 255) / 256), dim3(256), 0, st, chist, ncells, (const int*)cbsum);
+    hipLaunchKernelGGL(k_scan_add, dim3((ncells + 255) / 256), dim3(256), 0, st, chist, ncells, (const int*)cbsum,
+                       (size_t)0);
     LD_LAUNCHED();
     hipLaunchKernelGGL(k_coll_scatter, dim3(pb), dim3(256), 0, st, (int)n_, (const int*)ckey, (const int*)crank,
                        (const int*)chist, corder);

@@ -23,6 +23,17 @@ template <int N> struct Geo {
 // ({u_x, u_y, u_z, 0} and {v_x, v_y, v_z, n_stage}).
 template <typename T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
 
+// Batched launches: blockIdx.z selects the realization of an ensemble, and
+// every per-realization buffer lies rs bytes after the previous realization's
+// copy (rs = 0 with one realization).  Null pointers stay null.
+template <class X> __device__ __forceinline__ X* rsel(X* p, size_t rs) {
+  return p ? reinterpret_cast<X*>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * rs) : p;
+}
+// the same for a pointer that is never null
+template <class X> __device__ __forceinline__ X* rsel1(X* p, size_t rs) {
+  return reinterpret_cast<X*>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * rs);
+}
+
 // Loads/stores at a 32-bit element index from a uniform base pointer: the
 // byte offset stays 32-bit so the address is SGPR base + VGPR offset.
 template <typename X>
@@ -97,7 +108,7 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
 // DIR > 0: inverse times `scale`.  In place is allowed (each block owns its lines).
 template <typename T, int N, int M, int AXIS, int DIR>
 __global__ void __launch_bounds__(M * Plan<N>::N1)
-k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff) {
+k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff, size_t rs) {
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, js = JS<N, AXIS>::v;
   __shared__ C<T> lds[M * (N + 1)];
   const int m = threadIdx.x % M, n1 = threadIdx.x / M;
@@ -106,8 +117,8 @@ k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T
   const int nbx = gridDim.x;
   const int bx = (nbx % 8 == 0) ? (blockIdx.x % 8) * (nbx / 8) + blockIdx.x / 8 : blockIdx.x;
   const int q = qoff + bx * M + m;
-  const C<T>* src = in + (size_t)blockIdx.y * fstride;
-  C<T>* dst = out + (size_t)blockIdx.y * fstride;
+  const C<T>* src = rsel1(in, rs) + (size_t)blockIdx.y * fstride;
+  C<T>* dst = rsel1(out, rs) + (size_t)blockIdx.y * fstride;
   const uint32_t lb = (uint32_t)lineBase<N, AXIS>(q);
   C<T>* L = lds + m * (N + 1);
   C<T> a[N2];
@@ -235,7 +246,8 @@ template <int N, int P, int FIN, int FOUT> struct ZThreads {
 // Op::storeSpec].  Op::loadSpec supplies the spectral inputs.
 template <typename T, int N, int P, int FIN, int FOUT, class Op>
 __global__ void __launch_bounds__((ZThreads<N, P, FIN, FOUT>::v))
-k_z(Op op, const C<T>* __restrict__ tw, int lineoff) {
+k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
+  const Op op = op0.shifted(rs);
   constexpr int Nf = N / 2 + 1;
   constexpr int NREG = ZMax<FIN, FOUT>::v * P / 2;
   __shared__ C<T> lds[NREG * 2 * Nf];
@@ -269,6 +281,7 @@ k_z(Op op, const C<T>* __restrict__ tw, int lineoff) {
 template <typename T, int N, int P, int F>
 struct OpZInv {
   const C<T>* in; T* out; size_t fin, fout; T scale;
+  __device__ OpZInv shifted(size_t rs) const { OpZInv o = *this; o.in = rsel1(in, rs); o.out = rsel1(out, rs); return o; }
   __device__ C<T> loadSpec(int f, int line, int k) const { return in[f * fin + (size_t)line * (N / 2 + 1) + k]; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     for (int e = threadIdx.x; e < F * P * N; e += blockDim.x) {
@@ -282,6 +295,7 @@ struct OpZInv {
 template <typename T, int N, int P, int F>
 struct OpZFwd {
   const T* in; C<T>* out; size_t fin, fout;
+  __device__ OpZFwd shifted(size_t rs) const { OpZFwd o = *this; o.in = rsel1(in, rs); o.out = rsel1(out, rs); return o; }
   __device__ C<T> loadSpec(int, int, int) const { return C<T>{}; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     for (int e = threadIdx.x; e < F * P * N; e += blockDim.x) {
@@ -296,6 +310,7 @@ struct OpZFwd {
 template <typename T, int N, int P>
 struct OpZRound {
   C<T>* buf; size_t fs; T scale;
+  __device__ OpZRound shifted(size_t rs) const { OpZRound o = *this; o.buf = rsel1(buf, rs); return o; }
   __device__ C<T> loadSpec(int f, int line, int k) const { return buf[f * fs + (size_t)line * (N / 2 + 1) + k]; }
   __device__ void physical(const ZTile<T, N, P>& t, int) const {
     for (int e = threadIdx.x; e < 3 * P * N; e += blockDim.x) {
@@ -314,6 +329,9 @@ struct OpZRound {
 template <typename T, int N, int P>
 struct OpZCross {
   C<T>* w; size_t fs; T* u; V4<T>* gu; T scale;   // u planar (may be null) and/or node-interleaved (may be null)
+  __device__ OpZCross shifted(size_t rs) const {
+    OpZCross o = *this; o.w = rsel1(w, rs); o.u = rsel(u, rs); o.gu = rsel(gu, rs); return o;
+  }
   __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
@@ -350,6 +368,12 @@ __device__ __forceinline__ T nStage(T x, const double* __restrict__ clip) {
 template <typename T, int N, int P>
 struct OpZVel {
   const C<T>* w; C<T>* wout; size_t fs; T* v; V4<T>* gv; const T* n; const double* clip; T scale; T tpsg;
+  __device__ OpZVel shifted(size_t rs) const {
+    OpZVel o = *this;
+    o.w = rsel1(w, rs); o.wout = rsel1(wout, rs); o.v = rsel(v, rs); o.gv = rsel(gv, rs);
+    o.n = rsel1(n, rs); o.clip = rsel1(clip, rs);
+    return o;
+  }
   __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
@@ -383,6 +407,12 @@ struct OpZKn {
   const T* n0; T* nacc; T* nnext;
   T wacc, anext; int acc_init;
   int sum_target; double* partials;
+  __device__ OpZKn shifted(size_t rs) const {
+    OpZKn o = *this;
+    o.w = rsel1(w, rs); o.kn_out = rsel(kn_out, rs); o.n0 = rsel(n0, rs); o.nacc = rsel(nacc, rs);
+    o.nnext = rsel(nnext, rs); o.partials = rsel1(partials, rs);
+    return o;
+  }
   __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     double s_old = 0.0, s_new = 0.0;
@@ -430,6 +460,12 @@ template <typename T> struct RhsArgs {
   size_t fs;
   T nu_eff, tps, wacc, anext, scale;
   int acc_init;
+  __device__ RhsArgs shifted(size_t rs) const {
+    RhsArgs a = *this;
+    a.nl = rsel1(nl, rs); a.us = rsel1(us, rs); a.ubase = rsel1(ubase, rs); a.acc = rsel(acc, rs);
+    a.usnext = rsel(usnext, rs); a.ku = rsel(ku, rs); a.fac = rsel1(fac, rs);
+    return a;
+  }
 };
 
 // ---------------------------------------------- field-parallel x passes
@@ -446,7 +482,10 @@ template <typename T, int N, int M0> struct XM3 {
 // inverse-transforms u_f and w_f.
 template <typename T, int N, int M>
 __global__ void __launch_bounds__(3 * M * Plan<N>::N1)
-k_x_uw3(const C<T>* __restrict__ uk, C<T>* __restrict__ out, size_t fs, const C<T>* __restrict__ tw, T scale) {
+k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const C<T>* __restrict__ tw, T scale,
+        size_t rs) {
+  const C<T>* __restrict__ uk = rsel1(uk0, rs);
+  C<T>* __restrict__ out = rsel1(out0, rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
   __shared__ C<T> lds[3 * M * (N + 1)];
   const int f = threadIdx.x / G, loc = threadIdx.x % G;
@@ -498,7 +537,8 @@ k_x_uw3(const C<T>* __restrict__ uk, C<T>* __restrict__ out, size_t fs, const C<
 // inverse-transforms v_f.
 template <typename T, int N, int M>
 __global__ void __launch_bounds__(3 * M * Plan<N>::N1)
-k_x_rhs3(RhsArgs<T> A, const C<T>* __restrict__ tw) {
+k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
+  const RhsArgs<T> A = A0.shifted(rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
   __shared__ C<T> lds[3 * M * (N + 1)];
   const int f = threadIdx.x / G, loc = threadIdx.x % G;
@@ -590,7 +630,8 @@ template <typename T, int N, int M0> struct XM4 {
 };
 template <typename T, int N, int M>
 __global__ void __launch_bounds__(4 * M * Plan<N>::N1)
-k_x_kn4(C<T>* w, size_t fs, const C<T>* __restrict__ tw, T scale) {
+k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs) {
+  C<T>* w = rsel1(w0, rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
   __shared__ C<T> lds[4 * M * (N + 1)];
   const int g = threadIdx.x / G, loc = threadIdx.x % G;
@@ -636,7 +677,10 @@ k_x_kn4(C<T>* w, size_t fs, const C<T>* __restrict__ tw, T scale) {
 // (forced-dns-sm.py:601-610) + blow-up statistic max Re(u_hat) (:618).
 template <typename T, int N, int M>
 __global__ void __launch_bounds__(M * Plan<N>::N1)
-k_x_proj(const C<T>* w, C<T>* uk, size_t fs, const C<T>* __restrict__ tw, double* maxpart) {
+k_x_proj(const C<T>* w0, C<T>* uk0, size_t fs, const C<T>* __restrict__ tw, double* maxpart0, size_t rs) {
+  const C<T>* w = rsel1(w0, rs);
+  C<T>* uk = rsel1(uk0, rs);
+  double* maxpart = rsel1(maxpart0, rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
   constexpr int NT = M * N1;
   __shared__ C<T> lds[M * (N + 1)];
@@ -682,8 +726,11 @@ k_x_proj(const C<T>* w, C<T>* uk, size_t fs, const C<T>* __restrict__ tw, double
 //   fac[s] = f0 / (2 ek[s]) for s in shells (0 if |ek[s]| < 1e-10), times fscale
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
-k_shell(const C<T>* __restrict__ uk, size_t fs, int smax, const int* __restrict__ shells, int nsh,
-        double f0, double fscale, double norm0, double* __restrict__ ek, double* __restrict__ fac) {
+k_shell(const C<T>* __restrict__ uk0, size_t fs, int smax, const int* __restrict__ shells, int nsh,
+        double f0, double fscale, double norm0, double* __restrict__ ek0, double* __restrict__ fac0, size_t rs) {
+  const C<T>* __restrict__ uk = rsel1(uk0, rs);
+  double* __restrict__ ek = rsel1(ek0, rs);
+  double* __restrict__ fac = rsel1(fac0, rs);
   constexpr int NT = 256;
   __shared__ double sh[NT];
   constexpr int Nf = N / 2 + 1;
@@ -728,7 +775,9 @@ k_shell(const C<T>* __restrict__ uk, size_t fs, int smax, const int* __restrict_
 // (forced-dns-sm.py:281, :283), per block.
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_sum2(const T* __restrict__ x, size_t n, double* __restrict__ partials) {
+k_sum2(const T* __restrict__ x0, size_t n, double* __restrict__ partials0, size_t rs) {
+  const T* __restrict__ x = rsel1(x0, rs);
+  double* __restrict__ partials = rsel1(partials0, rs);
   constexpr int NT = 256;
   __shared__ double sh[NT];
   const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -747,7 +796,9 @@ k_sum2(const T* __restrict__ x, size_t n, double* __restrict__ partials) {
 
 // Final clip_zero means: clip = {sum/N^3, sum+/N^3, 1}.
 static __global__ void __launch_bounds__(256)
-k_clip_final(const double* __restrict__ partials, int nblk, double inv_n3, double* __restrict__ clip) {
+k_clip_final(const double* __restrict__ partials0, int nblk, double inv_n3, double* __restrict__ clip0, size_t rs) {
+  const double* __restrict__ partials = rsel1(partials0, rs);
+  double* __restrict__ clip = rsel1(clip0, rs);
   constexpr int NT = 256;
   __shared__ double sh[NT];
   double s0 = 0.0, s1 = 0.0;
@@ -760,7 +811,10 @@ k_clip_final(const double* __restrict__ partials, int nblk, double inv_n3, doubl
 
 // out = clip_zero(x) with x clipped in place when clip_x (forced-dns-sm.py:282-285).
 template <typename T>
-__global__ void k_clip_apply(T* x, T* out, size_t n, const double* __restrict__ clip, int clip_x) {
+__global__ void k_clip_apply(T* x0, T* out0, size_t n, const double* __restrict__ clip0, int clip_x, size_t rs) {
+  T* x = rsel1(x0, rs);
+  T* out = rsel1(out0, rs);
+  const double* __restrict__ clip = rsel1(clip0, rs);
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const T v = x[i];
     const T c = v > (T)0 ? v : (T)0;
@@ -771,7 +825,11 @@ __global__ void k_clip_apply(T* x, T* out, size_t n, const double* __restrict__ 
 
 // Blow-up test: flag |= max Re(u_hat) > thresh (forced-dns-sm.py:618); the
 // per-step maxima are kept for diagnostics.
-static __global__ void k_max_final(const double* __restrict__ part, int nblk, double thresh, int* flag, double* lastmax) {
+static __global__ void k_max_final(const double* __restrict__ part0, int nblk, double thresh, int* flag0,
+                                   double* lastmax0, size_t rs) {
+  const double* __restrict__ part = rsel1(part0, rs);
+  int* flag = rsel1(flag0, rs);
+  double* lastmax = rsel1(lastmax0, rs);
   constexpr int NT = 256;
   __shared__ double sh[NT];
   double m = -INFINITY;
@@ -800,7 +858,8 @@ __global__ void k_forcing_apply(const C<T>* __restrict__ uk, C<T>* __restrict__ 
 }
 
 template <typename T>
-__global__ void k_fill(T* x, size_t n, T v) {
+__global__ void k_fill(T* x0, size_t n, T v, size_t rs) {
+  T* x = rsel1(x0, rs);
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = v;
 }
 template <typename T>

@@ -196,6 +196,14 @@ template <typename T, typename PT> struct PStage {
   unsigned long long* wmax;    // out: largest W > 0 of the launch (bits of a positive double); zeroed before
   double ah, wh;               // a_s * h, w_s * h
   DropPhys P;
+  size_t rs;                   // realization stride in bytes (batched launches)
+  __device__ PStage shifted() const {
+    PStage a = *this;
+    a.x0 = rsel1(x0, rs); a.v0 = rsel1(v0, rs); a.sx = rsel1(sx, rs); a.sv = rsel1(sv, rs);
+    a.kx = rsel1(kx, rs); a.kv = rsel1(kv, rs); a.gu = rsel1(gu, rs); a.gv = rsel1(gv, rs);
+    a.dep = rsel1(dep, rs); a.flist = rsel1(flist, rs); a.fcount = rsel1(fcount, rs); a.wmax = rsel1(wmax, rs);
+    return a;
+  }
 };
 
 // One RK4 stage of every droplet: stage coordinate X0 + a h k_prev
@@ -207,9 +215,12 @@ template <typename T, typename PT> struct PStage {
 // wavefront load or atomic touches 16 droplets x 4 consecutive z cells (one
 // 16-byte segment per droplet).  Partial sums are combined across the four
 // lanes with xor shuffles.
-template <typename T, typename PT, int N>
+// BAT: batched launch (blockIdx.z = realization); the single-realization
+// instantiation skips the pointer offsets and their registers.
+template <typename T, typename PT, int N, bool BAT>
 __global__ void __launch_bounds__(256)
-k_particles_stage(PStage<T, PT> A) {
+k_particles_stage(PStage<T, PT> A0) {
+  const PStage<T, PT> A = BAT ? A0.shifted() : A0;
   // XCD-aware mapping: the 8 XCDs take consecutive blocks round-robin; give
   // each XCD one contiguous range of (spatially sorted) droplets so its L2
   // sees one region of the fields.
@@ -377,8 +388,12 @@ template <int N> struct Brick {
 
 template <typename T, int N>
 __global__ void __launch_bounds__(Brick<N>::NT)
-k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, int np,
-                const unsigned long long* __restrict__ wmax, T* __restrict__ fc, DropPhys P) {
+k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0, int np,
+                const unsigned long long* __restrict__ wmax0, T* __restrict__ fc0, DropPhys P, size_t rs) {
+  const double4* __restrict__ dep = rsel1(dep0, rs);
+  const int* __restrict__ start = rsel1(start0, rs);
+  const unsigned long long* __restrict__ wmax = rsel1(wmax0, rs);
+  T* __restrict__ fc = rsel1(fc0, rs);
   using Bk = Brick<N>;
   constexpr int S = Bk::S, NBr = Bk::NB, NBX = Bk::NBX, NR = Bk::NR, NT = Bk::NT, NZS = Bk::NZS, CH = Bk::CH;
   constexpr int TS = Bins<N>::TS;
@@ -521,8 +536,12 @@ k_brick_deposit(const double4* __restrict__ dep, const int* __restrict__ start, 
 // bin since the sort; runs after k_brick_deposit.
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
-k_fallback_deposit(const double4* __restrict__ dep, const int* __restrict__ flist, const int* __restrict__ fcount,
-                   T* fc, DropPhys P) {
+k_fallback_deposit(const double4* __restrict__ dep0, const int* __restrict__ flist0,
+                   const int* __restrict__ fcount0, T* fc0, DropPhys P, size_t rs) {
+  const double4* __restrict__ dep = rsel1(dep0, rs);
+  const int* __restrict__ flist = rsel1(flist0, rs);
+  const int* __restrict__ fcount = rsel1(fcount0, rs);
+  T* fc = rsel1(fc0, rs);
   const int nf = *fcount;
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
     const double4 d = dep[flist[f]];
@@ -620,8 +639,12 @@ static __global__ void k_axpy_rows(const double* __restrict__ x, const double* _
 
 template <int N>
 __global__ void __launch_bounds__(256)
-k_bin_count(const double* __restrict__ pos, int np, int cap, double L, double dx, int* __restrict__ key,
-            int* __restrict__ rank, int* __restrict__ hist) {
+k_bin_count(const double* __restrict__ pos0, int np, int cap, double L, double dx, int* __restrict__ key0,
+            int* __restrict__ rank0, int* __restrict__ hist0, size_t rs) {
+  const double* __restrict__ pos = rsel1(pos0, rs);
+  int* __restrict__ key = rsel1(key0, rs);
+  int* __restrict__ rank = rsel1(rank0, rs);
+  int* __restrict__ hist = rsel1(hist0, rs);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   int b[3];
@@ -664,7 +687,9 @@ __device__ __forceinline__ int blockExclusiveScan1024(int v, int* sh, int& total
 }
 
 static __global__ void __launch_bounds__(1024)
-k_scan_local(int* __restrict__ data, int n, int* __restrict__ bsum) {
+k_scan_local(int* __restrict__ data0, int n, int* __restrict__ bsum0, size_t rs) {
+  int* __restrict__ data = rsel1(data0, rs);
+  int* __restrict__ bsum = rsel1(bsum0, rs);
   __shared__ int sh[32];
   const int base = blockIdx.x * 4096 + threadIdx.x * 4;
   int v[4], s = 0;
@@ -681,7 +706,8 @@ k_scan_local(int* __restrict__ data, int n, int* __restrict__ bsum) {
 }
 
 static __global__ void __launch_bounds__(1024)
-k_scan_bsums(int* __restrict__ bsum, int nb) {
+k_scan_bsums(int* __restrict__ bsum0, int nb, size_t rs) {
+  int* __restrict__ bsum = rsel1(bsum0, rs);
   __shared__ int sh[32];
   const int base = threadIdx.x * 4;
   int v[4], s = 0;
@@ -697,7 +723,9 @@ k_scan_bsums(int* __restrict__ bsum, int nb) {
 }
 
 static __global__ void __launch_bounds__(256)
-k_scan_add(int* __restrict__ data, int n, const int* __restrict__ bsum) {
+k_scan_add(int* __restrict__ data0, int n, const int* __restrict__ bsum0, size_t rs) {
+  int* __restrict__ data = rsel1(data0, rs);
+  const int* __restrict__ bsum = rsel1(bsum0, rs);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) data[i] += bsum[i / 4096];
 }
@@ -705,10 +733,19 @@ k_scan_add(int* __restrict__ data, int n, const int* __restrict__ bsum) {
 // Scatter the droplet state to its sorted slot.
 template <typename PT>
 __global__ void __launch_bounds__(256)
-k_bin_scatter(int np, int cap, const int* __restrict__ key, const int* __restrict__ rank,
-              const int* __restrict__ start, const double* __restrict__ pos, const PT* __restrict__ vm,
-              const int64_t* __restrict__ id, double* __restrict__ pos2, PT* __restrict__ vm2,
-              int64_t* __restrict__ id2) {
+k_bin_scatter(int np, int cap, const int* __restrict__ key0, const int* __restrict__ rank0,
+              const int* __restrict__ start0, const double* __restrict__ pos0, const PT* __restrict__ vm0,
+              const int64_t* __restrict__ id0, double* __restrict__ pos20, PT* __restrict__ vm20,
+              int64_t* __restrict__ id20, size_t rs) {
+  const int* __restrict__ key = rsel1(key0, rs);
+  const int* __restrict__ rank = rsel1(rank0, rs);
+  const int* __restrict__ start = rsel1(start0, rs);
+  const double* __restrict__ pos = rsel1(pos0, rs);
+  const PT* __restrict__ vm = rsel1(vm0, rs);
+  const int64_t* __restrict__ id = rsel1(id0, rs);
+  double* __restrict__ pos2 = rsel1(pos20, rs);
+  PT* __restrict__ vm2 = rsel1(vm20, rs);
+  int64_t* __restrict__ id2 = rsel1(id20, rs);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int d = start[key[p]] + rank[p];

@@ -32,7 +32,12 @@ def _ptr(t: Optional[torch.Tensor]):
 
 
 class ForcedDNS:
-    def __init__(self, cfg: Config, max_particles: Optional[int] = None, device: str = "cuda"):
+    def __init__(self, cfg: Config, max_particles: Optional[int] = None, device: str = "cuda", batch: int = 1,
+                 graph: bool = False, autotune: bool = False):
+        """batch > 1: an ensemble stepped with one launch per kernel (only the
+        resident-state methods accept it); graph: advance() replays a captured
+        HIP graph of one step; autotune: pick the y-pass launch geometry by
+        timing at creation.  None of the three changes the results."""
         if not torch.cuda.is_available():
             raise RuntimeError("ForcedDNS needs a GPU (libld.so has no CPU path)")
         self.cfg = cfg
@@ -61,6 +66,11 @@ class ForcedDNS:
         p.factor = cfg.factor
         p.growthfactor = cfg.growthfactor
         p.max_particles = int(cfg.n_particles if max_particles is None else max_particles)
+        p.batch = int(batch)
+        p.use_graph = 1 if graph else 0
+        p.autotune = 1 if autotune else 0
+        self.batch = max(1, int(batch))
+        self.np = 0
         self.params = p
         self.max_particles = p.max_particles
         ctx = ctypes.c_void_p()
@@ -209,15 +219,32 @@ class ForcedDNS:
         return uk, n
 
     # ------------------------------------------------------------ resident state (fast path)
+    def _bshape(self, shape):
+        return shape if self.batch == 1 else (self.batch,) + tuple(shape)
+
     def set_state(self, uk, n, coord=None):
-        self._chk(uk, self.cdtype, self.cshape(3), "uk")
-        self._chk(n, self.rdtype, self.rshape(), "n")
-        coord, npart = self._rows(coord)
+        """Load the resident state.  With batch B > 1 the arrays carry a leading
+        realization axis: uk (B, 3, N, N, Nf), n (B, N, N, N), coord (B, np, 7)."""
+        self._chk(uk, self.cdtype, self._bshape(self.cshape(3)), "uk")
+        self._chk(n, self.rdtype, self._bshape(self.rshape()), "n")
+        if self.batch > 1 and coord is not None:
+            if coord.dim() != 3 or coord.shape[0] != self.batch:
+                raise ValueError("ensemble droplet rows must be (B, np, 7)")
+            coord, npart = self._rows(coord.reshape(-1, 7))
+            npart //= self.batch
+        else:
+            coord, npart = self._rows(coord)
         self._stream()
         _lib.check(self.lib.ld_state_set(self.ctx, _ptr(uk), _ptr(n), _ptr(coord), npart))
         self.np = npart
 
     def get_state(self, uk=None, n=None, coord=None):
+        if uk is not None:
+            self._chk(uk, self.cdtype, self._bshape(self.cshape(3)), "uk")
+        if n is not None:
+            self._chk(n, self.rdtype, self._bshape(self.rshape()), "n")
+        if coord is not None:
+            self._chk(coord, torch.float64, self._bshape((self.np, 7)), "coord")
         self._stream()
         _lib.check(self.lib.ld_state_get(self.ctx, _ptr(uk), _ptr(n), _ptr(coord)))
         return uk, n, coord
@@ -225,6 +252,22 @@ class ForcedDNS:
     def advance(self, h, nsteps: int = 1):
         self._stream()
         _lib.check(self.lib.ld_advance(self.ctx, float(h), int(nsteps)))
+
+    def tuning(self):
+        """(y-pass lines per block, [median us with M and with M/2 lines])."""
+        yl = ctypes.c_int32()
+        us = (ctypes.c_double * 2)()
+        _lib.check(self.lib.ld_tuning(self.ctx, ctypes.byref(yl), us))
+        return yl.value, [us[0], us[1]]
+
+    def status_all(self):
+        """(last max Re(u_hat), blown-up flag) per realization, as numpy arrays."""
+        import numpy as np
+        lm = (ctypes.c_double * self.batch)()
+        bl = (ctypes.c_int32 * self.batch)()
+        self._stream()
+        _lib.check(self.lib.ld_status_all(self.ctx, lm, bl))
+        return np.array(lm[:]), np.array(bl[:], dtype=bool)
 
     # ------------------------------------------------------------ diagnostics (save(), plot.py)
     def spectrum(self, fk: torch.Tensor, weight: float = 0.5) -> torch.Tensor:

@@ -362,6 +362,106 @@ def test_advance_bitwise_reproducible(gpu, prec):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_ensemble_matches_single_runs(gpu, prec):
+    """A batched ensemble (one launch per kernel for all realizations) evolves
+    each realization bit for bit as a context holding it alone."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = preset("c1").replace(precision=prec, N=32, n_particles=3000)
+    B, steps = 3, 4
+    uks, ns, cs, dts = [], [], [], []
+    for b in range(B):
+        rng = np.random.default_rng(100 + b)
+        uk, n, u = ic.initial_flow(cfg, rng)
+        dts.append(ic.cfl_dt(cfg, u))
+        coord, _ = ic.initial_droplets(cfg, rng)
+        uks.append(uk); ns.append(n); cs.append(coord)
+    dt = min(dts)
+    e = ForcedDNS(cfg, batch=B)
+    uke = _t(np.stack(uks), gpu, e.cdtype)
+    ne = _t(np.stack(ns), gpu, e.rdtype)
+    ce = _t(np.stack(cs), gpu, torch.float64)
+    e.set_state(uke, ne, ce)
+    e.advance(dt, steps)
+    e.get_state(uke, ne, ce)
+    lm, bl = e.status_all()
+    torch.cuda.synchronize()
+    assert lm.shape == (B,) and not bl.any()
+    with pytest.raises(Exception):
+        e.forcing(uke[0], torch.zeros_like(uke[0]))        # single-realization entry points refuse
+    e.close()
+    for b in range(B):
+        d = ForcedDNS(cfg)
+        ukg, ng, cg = _t(uks[b], gpu, d.cdtype), _t(ns[b], gpu, d.rdtype), _t(cs[b], gpu, torch.float64)
+        d.set_state(ukg, ng, cg)
+        d.advance(dt, steps)
+        d.get_state(ukg, ng, cg)
+        torch.cuda.synchronize()
+        for a, ref in ((uke[b], ukg), (ne[b], ng), (ce[b], cg)):
+            assert np.array_equal(a.cpu().numpy().view(np.uint8), ref.cpu().numpy().view(np.uint8))
+        d.close()
+
+
+def _c1_state(gpu, prec, npart, seed=11):
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    cfg = preset("c1").replace(precision=prec, n_particles=npart)
+    rng = np.random.default_rng(seed)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    coord, _ = ic.initial_droplets(cfg, rng)
+    return cfg, uk, n, coord, ic.cfl_dt(cfg, u)
+
+
+def test_graph_replays_match_eager(gpu):
+    """advance() through captured HIP graphs equals eager launches bit for bit,
+    replay after replay (odd step counts alternate the droplet buffers)."""
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg, uk, n, coord, dt = _c1_state(gpu, "f32", 8000)
+    ctxs = [ForcedDNS(cfg), ForcedDNS(cfg, graph=True)]
+    st = []
+    for d in ctxs:
+        t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+        d.set_state(*t)
+        st.append(t)
+    for k, nsteps in enumerate((1, 2, 3, 1)):
+        for d, t in zip(ctxs, st):
+            d.advance(dt, nsteps)
+            d.get_state(*t)
+        torch.cuda.synchronize()
+        for a, b in zip(st[0], st[1]):
+            assert np.array_equal(a.cpu().numpy().view(np.uint8), b.cpu().numpy().view(np.uint8)), f"replay {k}"
+        x = torch.ones(1 << 20, device=gpu).sum()        # eager work between replays
+        assert float(x) == float(1 << 20)
+    for d in ctxs:
+        d.close()
+
+
+def test_autotune_keeps_results(gpu):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg, uk, n, coord, dt = _c1_state(gpu, "f32", 4000)
+    outs = []
+    for tune in (False, True):
+        d = ForcedDNS(cfg, autotune=tune)
+        t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+        d.set_state(*t)
+        d.advance(dt, 2)
+        d.get_state(*t)
+        torch.cuda.synchronize()
+        outs.append([x.cpu().numpy() for x in t])
+        if tune:
+            yl, us = d.tuning()
+            assert yl in (16, 8) and min(us) > 0
+        d.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
 def test_c1_200_steps_against_oracle_golden(gpu):
     """The 64^3 FP64 workload (10^4 droplets, 200 steps) against the committed
     CPU-oracle run (tests/golden/make_c1_golden.py): 1e-10 relative on the
