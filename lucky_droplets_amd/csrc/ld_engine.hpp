@@ -77,6 +77,7 @@ struct Engine final : EngineBase {
   C<T>* tw = nullptr;
   hipStream_t st2 = nullptr;        // density/droplet stream (created here)
   bool serial = false;              // no overlap of the flow and density chains
+  unsigned pstage_lds = 0;          // dynamic LDS bytes per block of the droplet stage kernel
   hipEvent_t evA = nullptr, evB = nullptr;
   C<T>*uk = nullptr, *us = nullptr, *acc = nullptr, *w6 = nullptr, *wn = nullptr;
   T* ubuf[2] = {nullptr, nullptr};       // planar physical u (API / plain paths)
@@ -217,6 +218,10 @@ struct Engine final : EngineBase {
     // LD_SERIAL=1: the flow stream waits for each stage's density chain, so the
     // two chains never run concurrently (per-kernel timing without overlap)
     if (const char* e = getenv("LD_SERIAL")) serial = atoi(e) != 0;
+    if (const char* e = getenv("LD_PSTAGE_LDS")) {
+      const int v = atoi(e);
+      if (v >= 0 && v <= 65536) pstage_lds = (unsigned)v;
+    }
     if (const char* e = getenv("LD_SLABS")) {
       const int v = atoi(e);
       if (v >= 1 && N % v == 0 && (N / v) % M == 0) nslab = v;
@@ -516,10 +521,12 @@ struct Engine final : EngineBase {
       // 4 lanes per droplet; block count a multiple of 8 for the XCD mapping
       const unsigned nb = (unsigned)(((4 * np + 255) / 256 + 7) / 8 * 8);
       LD_TRY(prof_mark(LD_K_DROPLETS, st2));
+      // pstage_lds: dynamic LDS reserved per block, which caps the blocks per CU
+      // (and so the droplets in flight per XCD, whose stencils share its L2)
       if (B == 1)
-        hipLaunchKernelGGL((k_particles_stage<T, PT, N, false>), dim3(nb), dim3(256), 0, st2, ps);
+        hipLaunchKernelGGL((k_particles_stage<T, PT, N, false>), dim3(nb), dim3(256), pstage_lds, st2, ps);
       else
-        hipLaunchKernelGGL((k_particles_stage<T, PT, N, true>), dim3(nb, 1, B), dim3(256), 0, st2, ps);
+        hipLaunchKernelGGL((k_particles_stage<T, PT, N, true>), dim3(nb, 1, B), dim3(256), pstage_lds, st2, ps);
       LD_LAUNCHED();
       LD_TRY(prof_mark(LD_K_DROPLETS, st2));
       if (sorted_ok) {

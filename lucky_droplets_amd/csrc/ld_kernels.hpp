@@ -26,12 +26,15 @@ template <typename T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
 // Batched launches: blockIdx.z selects the realization of an ensemble, and
 // every per-realization buffer lies rs bytes after the previous realization's
 // copy (rs = 0 with one realization).  Null pointers stay null.
-template <class X> __device__ __forceinline__ X* rsel(X* p, size_t rs) {
-  return p ? reinterpret_cast<X*>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * rs) : p;
-}
-// the same for a pointer that is never null
+// (byte arithmetic on char pointers, not integers, so the compiler still
+// sees global-memory pointers and emits global_* rather than flat_* accesses)
 template <class X> __device__ __forceinline__ X* rsel1(X* p, size_t rs) {
-  return reinterpret_cast<X*>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * rs);
+  using Byte = std::conditional_t<std::is_const<X>::value, const char, char>;
+  return reinterpret_cast<X*>(reinterpret_cast<Byte*>(p) + (size_t)blockIdx.z * rs);
+}
+// the same for an optional pointer: null stays null
+template <class X> __device__ __forceinline__ X* rsel(X* p, size_t rs) {
+  return p ? rsel1(p, rs) : p;
 }
 
 // Loads/stores at a 32-bit element index from a uniform base pointer: the

@@ -7,6 +7,7 @@
 // (vm[c*cap + p], c = vx, vy, vz, m/M0).  One thread per droplet.
 #pragma once
 #include "ld_kernels.hpp"
+#include <type_traits>
 
 namespace ld {
 
@@ -166,8 +167,10 @@ __device__ __forceinline__ void dropletRhs(const DropPhys& P, const double vp[3]
 // that lanes of a wavefront gather from and deposit into the same cache
 // lines.  Bins are cubes of BS = max(2, N/128) grid cells; bins are numbered
 // tile-major (tiles of TS^3 bins = TC^3 cells, the deposit bricks), so the
-// droplets of one tile are contiguous and a run of gather blocks covers a
-// compact region of the fields.
+// droplets of one tile are contiguous; tiles run along z in columns (long
+// contiguous z-runs of the fields), and the columns follow a Morton (Z-order)
+// curve in (x, y), so the columns a run of gather blocks covers are mostly
+// neighbours and share their halo nodes in one L2.
 template <int N> struct Bins {
   static constexpr int BS = (N / 128 > 2) ? N / 128 : 2;
   static constexpr int B = N / BS;
@@ -176,9 +179,18 @@ template <int N> struct Bins {
   static constexpr int TS = TC / BS;                 // tile side in bins
   static constexpr int NTt = B / TS;                 // tiles per side
   static_assert(TC % BS == 0 && B % TS == 0, "bin tiles");
-  // key of bin (bx, by, bz)
+  static_assert((NTt & (NTt - 1)) == 0 && NTt <= 256, "tiles per side: a power of two");
+  // bits of v spread to every second position (v < 256)
+  static __device__ __forceinline__ int spread2(int v) {
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r |= ((v >> i) & 1) << (2 * i);
+    return r;
+  }
+  // key of bin (bx, by, bz): tile columns along z, the columns in Morton order
+  // of (tile x, tile y); x-major inside the tile
   static __device__ __forceinline__ int key(int bx, int by, int bz) {
-    const int tile = ((bx / TS) * NTt + by / TS) * NTt + bz / TS;
+    const int tile = ((spread2(bx / TS) << 1) | spread2(by / TS)) * NTt + bz / TS;
     return tile * (TS * TS * TS) + ((bx % TS) * TS + by % TS) * TS + bz % TS;
   }
 };
@@ -247,54 +259,62 @@ k_particles_stage(PStage<T, PT> A0) {
   if (A.stage > 0) m = (double)(PT)(A.v0[3 * cap + pp] + (PT)A.ah * A.kv[3 * cap + pp]);
   const double L = A.P.L, dx = A.P.dx, pi_2dx = A.P.pi_2dx;
   const double xw = wrapL(xs[0], L), yw = wrapL(xs[1], L), zw = wrapL(xs[2], L);
-  // one sincos per axis (cosW4); node i0 + a has weight (1 +- {cos, sin})/4
+  // one sincos per axis (cosW4); node i0 + a has weight (1 +- {cos, sin})/4.
+  // The angles and the sincos are FP64; the weights and the gather sums are
+  // kept in the field precision GT (FP32 fields: FP32 sums of 64 terms).
+  using GT = std::conditional_t<sizeof(T) == 4, float, double>;
   const int i0 = (int)floor(xw / dx) - 1, k0 = (int)floor(zw / dx) - 1;
   double xsn, xcs, zsn, zcs;
   sincos((xw - (double)i0 * dx) * pi_2dx, &xsn, &xcs);
   sincos((zw - (double)k0 * dx) * pi_2dx, &zsn, &zcs);
-  double wy[4];
+  GT wy[4];
   int iy[4];
-  const int j0 = cosW4(yw, dx, pi_2dx, wy);
+  {
+    double wyd[4];
+    const int j0 = cosW4(yw, dx, pi_2dx, wyd);
 #pragma unroll
-  for (int b = 0; b < 4; ++b) iy[b] = wrapIdx<N>(j0 + b);
-  const double wz = 0.25 * (1.0 + ((c & 2) ? -1.0 : 1.0) * ((c & 1) ? zsn : zcs));
+    for (int b = 0; b < 4; ++b) { wy[b] = (GT)wyd[b]; iy[b] = wrapIdx<N>(j0 + b); }
+  }
+  const GT wz = (GT)(0.25 * (1.0 + ((c & 2) ? -1.0 : 1.0) * ((c & 1) ? zsn : zcs)));
+  const GT gxs = (GT)xsn, gxc = (GT)xcs;
   const int iz = wrapIdx<N>(k0 + c);
   // gather: this lane's z node, all (x, y) nodes; two 16-byte records per node
-  double I[7];
+  GT Ig[7];
 #pragma unroll
-  for (int k = 0; k < 7; ++k) I[k] = 0.0;
+  for (int k = 0; k < 7; ++k) Ig[k] = (GT)0;
   if (live) {
 #pragma unroll 1
     for (int a = 0; a < 4; ++a) {
-      const double wx = 0.25 * (1.0 + ((a & 2) ? -1.0 : 1.0) * ((a & 1) ? xsn : xcs));
+      const GT wx = (GT)0.25 * ((GT)1 + ((a & 2) ? (GT)-1 : (GT)1) * ((a & 1) ? gxs : gxc));
       const size_t plane = (size_t)wrapIdx<N>(i0 + a) * N;
-      double t[7];
+      GT t[7];
 #pragma unroll
-      for (int k = 0; k < 7; ++k) t[k] = 0.0;
+      for (int k = 0; k < 7; ++k) t[k] = (GT)0;
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const size_t o = (plane + iy[b]) * N + iz;
-        const double w = wy[b];
+        const GT w = wy[b];
         const V4<T> U = A.gu[o];
         const V4<T> V = A.gv[o];
-        t[0] += (double)U.x * w;
-        t[1] += (double)U.y * w;
-        t[2] += (double)U.z * w;
-        t[3] += (double)V.x * w;
-        t[4] += (double)V.y * w;
-        t[5] += (double)V.z * w;
-        t[6] += (double)V.w * w;
+        t[0] += (GT)U.x * w;
+        t[1] += (GT)U.y * w;
+        t[2] += (GT)U.z * w;
+        t[3] += (GT)V.x * w;
+        t[4] += (GT)V.y * w;
+        t[5] += (GT)V.z * w;
+        t[6] += (GT)V.w * w;
       }
 #pragma unroll
-      for (int k = 0; k < 7; ++k) I[k] += t[k] * wx;
+      for (int k = 0; k < 7; ++k) Ig[k] += t[k] * wx;
     }
   }
+  double I[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
-    double v = I[k] * wz;
+    GT v = Ig[k] * wz;
     v += __shfl_xor(v, 1, 64);
     v += __shfl_xor(v, 2, 64);
-    I[k] = v;
+    I[k] = (double)v;
   }
   double rhs[7], W = 0.0;
   if (live) dropletRhs(A.P, vp, m, I, rhs, W);
