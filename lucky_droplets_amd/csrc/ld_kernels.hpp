@@ -244,24 +244,35 @@ template <int N, int P, int FIN, int FOUT> struct ZThreads {
   static constexpr int v = ZMax<FIN, FOUT>::v * P / 2 * Plan<N>::N1;
 };
 
+// Two consecutive spectral entries, moved as one 16-byte (FP32) access.
+template <typename T> struct alignas(2 * sizeof(C<T>)) C2 { C<T> a, b; };
+
 // Generic z-pass kernel: [inverse of FIN spectral fields] -> Op::physical on
-// the P real lines of every field -> [forward of FOUT fields, stored through
-// Op::storeSpec].  Op::loadSpec supplies the spectral inputs.
+// the P real lines of every field -> [forward of FOUT fields].  Spectral
+// field f of the Op is read from Op::specIn(f) and written to Op::specOut(f)
+// ([x][y][kz] layout).  The P lines of a block are one contiguous run of
+// P*Nf entries per field, kept in LDS in the same order (the regions of
+// consecutive line pairs are adjacent), so they move as pairs of entries;
+// line0 * Nf is even (P and the line offset are even), so every pair is
+// aligned.
 template <typename T, int N, int P, int FIN, int FOUT, class Op>
 __global__ void __launch_bounds__((ZThreads<N, P, FIN, FOUT>::v))
 k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
+  static_assert(P % 2 == 0, "pairs of lines");
   const Op op = op0.shifted(rs);
   constexpr int Nf = N / 2 + 1;
   constexpr int NREG = ZMax<FIN, FOUT>::v * P / 2;
-  __shared__ C<T> lds[NREG * 2 * Nf];
+  constexpr int NE = P * Nf;        // entries per field of a block
+  __shared__ C2<T> lds2[NREG * Nf];
+  C<T>* lds = &lds2[0].a;
   ZTile<T, N, P> tile{lds};
   const int line0 = lineoff + blockIdx.x * P;
+  const size_t g0 = (size_t)line0 * Nf;
   const int nt = blockDim.x;
   if constexpr (FIN > 0) {
-    for (int e = threadIdx.x; e < FIN * P * Nf; e += nt) {
-      const int f = e / (P * Nf), rem = e % (P * Nf);
-      const int p = rem / Nf, k = rem % Nf;
-      tile.spec(f, p, k) = op.loadSpec(f, line0 + p, k);
+    for (int e = threadIdx.x; e < FIN * (NE / 2); e += nt) {
+      const int f = e / (NE / 2), v = e % (NE / 2);
+      lds2[f * (NE / 2) + v] = reinterpret_cast<const C2<T>*>(op.specIn(f) + g0)[v];
     }
     __syncthreads();
     zInverse<T, N, P, FIN>(tile, tw);
@@ -270,10 +281,12 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
   __syncthreads();
   if constexpr (FOUT > 0) {
     zForward<T, N, P, FOUT>(tile, tw);
-    for (int e = threadIdx.x; e < FOUT * P * Nf; e += nt) {
-      const int f = e / (P * Nf), rem = e % (P * Nf);
-      const int p = rem / Nf, k = rem % Nf;
-      op.storeSpec(f, line0 + p, k, zSplit<T, N, P>(tile, f, p, k));
+    for (int e = threadIdx.x; e < FOUT * (NE / 2); e += nt) {
+      const int f = e / (NE / 2), q = 2 * (e % (NE / 2));
+      C2<T> v;
+      v.a = zSplit<T, N, P>(tile, f, q / Nf, q % Nf);
+      v.b = zSplit<T, N, P>(tile, f, (q + 1) / Nf, (q + 1) % Nf);
+      reinterpret_cast<C2<T>*>(op.specOut(f) + g0)[q / 2] = v;
     }
   }
 }
@@ -285,28 +298,28 @@ template <typename T, int N, int P, int F>
 struct OpZInv {
   const C<T>* in; T* out; size_t fin, fout; T scale;
   __device__ OpZInv shifted(size_t rs) const { OpZInv o = *this; o.in = rsel1(in, rs); o.out = rsel1(out, rs); return o; }
-  __device__ C<T> loadSpec(int f, int line, int k) const { return in[f * fin + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ const C<T>* specIn(int f) const { return in + f * fin; }
+  __device__ C<T>* specOut(int) const { return nullptr; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     for (int e = threadIdx.x; e < F * P * N; e += blockDim.x) {
       const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
       out[f * fout + (size_t)(line0 + p) * N + j] = t.re(f, p, j) * scale;
     }
   }
-  __device__ void storeSpec(int, int, int, C<T>) const {}
 };
 // Plain r2c along z of F fields (first pass of rfft_mpi, forced-dns-sm.py:202).
 template <typename T, int N, int P, int F>
 struct OpZFwd {
   const T* in; C<T>* out; size_t fin, fout;
   __device__ OpZFwd shifted(size_t rs) const { OpZFwd o = *this; o.in = rsel1(in, rs); o.out = rsel1(out, rs); return o; }
-  __device__ C<T> loadSpec(int, int, int) const { return C<T>{}; }
+  __device__ const C<T>* specIn(int) const { return nullptr; }
+  __device__ C<T>* specOut(int f) const { return out + f * fout; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     for (int e = threadIdx.x; e < F * P * N; e += blockDim.x) {
       const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
       t.re(f, p, j) = in[f * fin + (size_t)(line0 + p) * N + j];
     }
   }
-  __device__ void storeSpec(int f, int line, int k, C<T> v) const { out[f * fout + (size_t)line * (N / 2 + 1) + k] = v; }
 };
 // c2r followed by r2c of 3 fields (the reality round trip of
 // forced-dns-sm.py:597-603), in place.
@@ -314,14 +327,14 @@ template <typename T, int N, int P>
 struct OpZRound {
   C<T>* buf; size_t fs; T scale;
   __device__ OpZRound shifted(size_t rs) const { OpZRound o = *this; o.buf = rsel1(buf, rs); return o; }
-  __device__ C<T> loadSpec(int f, int line, int k) const { return buf[f * fs + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ const C<T>* specIn(int f) const { return buf + f * fs; }
+  __device__ C<T>* specOut(int f) const { return buf + f * fs; }
   __device__ void physical(const ZTile<T, N, P>& t, int) const {
     for (int e = threadIdx.x; e < 3 * P * N; e += blockDim.x) {
       const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
       t.re(f, p, j) *= scale;
     }
   }
-  __device__ void storeSpec(int f, int line, int k, C<T> v) const { buf[f * fs + (size_t)line * (N / 2 + 1) + k] = v; }
 };
 
 // Velocity and vorticity to physical space, u stored, then the rotational
@@ -335,7 +348,8 @@ struct OpZCross {
   __device__ OpZCross shifted(size_t rs) const {
     OpZCross o = *this; o.w = rsel1(w, rs); o.u = rsel(u, rs); o.gu = rsel(gu, rs); return o;
   }
-  __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ const C<T>* specIn(int f) const { return w + f * fs; }
+  __device__ C<T>* specOut(int f) const { return w + f * fs; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
     for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
@@ -350,7 +364,6 @@ struct OpZCross {
       t.re(2, p, j) = wy * ux - wx * uy;
     }
   }
-  __device__ void storeSpec(int f, int line, int k, C<T> v) const { w[f * fs + (size_t)line * (N / 2 + 1) + k] = v; }
 };
 
 // Stage density as the stage sees it: the reference stores clip_zero(x) =
@@ -377,7 +390,8 @@ struct OpZVel {
     o.n = rsel1(n, rs); o.clip = rsel1(clip, rs);
     return o;
   }
-  __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ const C<T>* specIn(int f) const { return w + f * fs; }
+  __device__ C<T>* specOut(int f) const { return wout + f * fs; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
     for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
@@ -393,7 +407,6 @@ struct OpZVel {
       t.re(2, p, j) = nn * vz;
     }
   }
-  __device__ void storeSpec(int f, int line, int k, C<T> v_) const { wout[f * fs + (size_t)line * (N / 2 + 1) + k] = v_; }
 };
 
 // Density RHS kn to physical space (last pass of forced-dns-sm.py:330) fused
@@ -416,7 +429,8 @@ struct OpZKn {
     o.nnext = rsel(nnext, rs); o.partials = rsel1(partials, rs);
     return o;
   }
-  __device__ C<T> loadSpec(int f, int line, int k) const { return w[f * fs + (size_t)line * (N / 2 + 1) + k]; }
+  __device__ const C<T>* specIn(int f) const { return w + f * fs; }
+  __device__ C<T>* specOut(int) const { return nullptr; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     double s_old = 0.0, s_new = 0.0;
     for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
@@ -447,7 +461,6 @@ struct OpZKn {
       if (threadIdx.x == 0) partials[2 * blk + 1] = b;
     }
   }
-  __device__ void storeSpec(int, int, int, C<T>) const {}
 };
 
 // ======================================================== fused x passes
