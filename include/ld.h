@@ -211,6 +211,38 @@ int ld_pair_stats(ld_ctx* ctx, const double* rows, int64_t np, double rmax, int3
 int ld_coalesce(ld_ctx* ctx, double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
                 int64_t* nevents);
 
+/* ---------------------------------------------------------- learned subgrid form */
+/* A periodic 3-D CNN on an N^3 grid: nlayers 3x3x3 convolutions with bias,
+ * ReLU between them, channels[0] inputs, channels[nlayers] outputs (each
+ * 1..32), hidden widths 16 or 32; bf16 weights and activations, FP32
+ * accumulation (MFMA), FP32 master parameters trained with Adam on a
+ * mean-squared-error loss.  Fields are rows x[node][c], node = (i N + j) N + k,
+ * FP32, device memory.  Parameter layout (FP32, device): per layer
+ * W[tap][co][ci] (tap = (dx+1) 9 + (dy+1) 3 + (dz+1), offsets -1..1, channel
+ * counts padded to 16 or 32, padded entries kept 0) then b[co].  No
+ * counterpart in the reference. */
+#define LD_CNN_MAX_LAYERS 6
+typedef struct ld_cnn_params {
+  int32_t N;                                  /* grid points per side, power of two, 4..512 */
+  int32_t nlayers;                            /* 1..LD_CNN_MAX_LAYERS */
+  int32_t channels[LD_CNN_MAX_LAYERS + 1];    /* channel count at each layer boundary */
+} ld_cnn_params;
+typedef struct ld_cnn ld_cnn;
+int ld_cnn_create(const ld_cnn_params* p, ld_cnn** out);
+int ld_cnn_destroy(ld_cnn* c);
+int ld_cnn_set_stream(ld_cnn* c, void* hip_stream);
+int ld_cnn_nparams(ld_cnn* c, int64_t* n);                 /* n: host */
+int ld_cnn_set_params(ld_cnn* c, const float* params);     /* also resets the Adam moments */
+int ld_cnn_get_params(ld_cnn* c, float* params);
+/* y[node][channels[nlayers]] = net(x[node][channels[0]]) */
+int ld_cnn_forward(ld_cnn* c, const float* x, float* y);
+/* loss = mean over nodes and output channels of (net(x) - target)^2 (host
+ * double); grads (may be null) receives d loss / d params. */
+int ld_cnn_grad(ld_cnn* c, const float* x, const float* target, float* grads, double* loss);
+/* ld_cnn_grad followed by one Adam step on the parameters. */
+int ld_cnn_train_step(ld_cnn* c, const float* x, const float* target, float lr, float beta1, float beta2, float eps,
+                      double* loss);
+
 #ifdef __cplusplus
 }
 #endif

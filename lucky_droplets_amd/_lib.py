@@ -46,6 +46,16 @@ class LdParams(ctypes.Structure):
     ]
 
 
+class LdCnnParams(ctypes.Structure):
+    """Mirror of ``ld_cnn_params`` in include/ld.h."""
+
+    _fields_ = [
+        ("N", ctypes.c_int32),
+        ("nlayers", ctypes.c_int32),
+        ("channels", ctypes.c_int32 * 7),
+    ]
+
+
 class LdError(RuntimeError):
     """An LD_ERR_* status from libld.so."""
 
@@ -92,6 +102,16 @@ _SIGS = {
     "ld_pair_stats": [_VP, _VP, ctypes.c_int64, ctypes.c_double, ctypes.c_int32, _VP, _VP, _VP],
     "ld_status_all": [_VP, _VP, _VP],
     "ld_tuning": [_VP, _VP, _VP],
+    "ld_cnn_create": [ctypes.POINTER(LdCnnParams), ctypes.POINTER(_VP)],
+    "ld_cnn_destroy": [_VP],
+    "ld_cnn_set_stream": [_VP, _VP],
+    "ld_cnn_nparams": [_VP, ctypes.POINTER(ctypes.c_int64)],
+    "ld_cnn_set_params": [_VP, _VP],
+    "ld_cnn_get_params": [_VP, _VP],
+    "ld_cnn_forward": [_VP, _VP, _VP],
+    "ld_cnn_grad": [_VP, _VP, _VP, _VP, ctypes.POINTER(ctypes.c_double)],
+    "ld_cnn_train_step": [_VP, _VP, _VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                          ctypes.POINTER(ctypes.c_double)],
 }
 EXPORTS = tuple(_SIGS) + ("ld_version",)
 

@@ -62,6 +62,9 @@ def build(jobs: int | None = None, force: bool = True, verbose: bool = False) ->
     api_obj = BUILD / "ld_api.o"
     tasks.append((api_obj, [HIPCC, *BASE_FLAGS, "-c", str(CSRC / "ld_api.hip"), "-o", str(api_obj)],
                   headers + [CSRC / "ld_api.hip"]))
+    cnn_obj = BUILD / "ld_cnn.o"
+    tasks.append((cnn_obj, [HIPCC, *BASE_FLAGS, "-c", str(CSRC / "ld_cnn.hip"), "-o", str(cnn_obj)],
+                  headers + [CSRC / "ld_cnn.hip"]))
     todo = [(o, c) for (o, c, deps) in tasks if force or _sources_newer(o, deps)]
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

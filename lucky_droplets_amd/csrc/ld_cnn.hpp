@@ -1,0 +1,243 @@
+// Learned subgrid form: a small periodic 3-D CNN on a coarse grid, trained and
+// run with hand-written bf16 MFMA kernels (FP32 accumulation).  No
+// counterpart in the reference.
+//
+// Layout: activations channels-last, A[node][C] in bf16, node = (x N + y) N + z,
+// C a multiple of 16; weights per layer W[tap][co][ci] (and the transposed
+// copy Wt[tap][ci][co] for the data gradient), tap = (dx+1) 9 + (dy+1) 3 +
+// (dz+1) for offsets in {-1, 0, 1}; periodic neighbours.
+//
+// A 3x3x3 convolution is an implicit GEMM: M = nodes, K = 27 taps x CI,
+// N = CO.  One wavefront computes 16 nodes x CO outputs with
+// __builtin_amdgcn_mfma_f32_16x16x16bf16_1k, whose operand layout is
+//   A (16x16, M x K): lane l holds A[l % 16][4 (l / 16) + i], i = 0..3
+//   B (16x16, K x N): lane l holds B[4 (l / 16) + i][l % 16]
+//   D (16x16):        lane l holds D[4 (l / 16) + i][l % 16]
+// so a lane reads 4 consecutive channels (8 bytes) of one node.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ld {
+namespace cnn {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+// FP32 -> bf16, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ bf16 f2b(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float b2f(bf16 b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ int nbr(int node, int tap, int N, bool flip) {
+  int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+  if (flip) { dx = -dx; dy = -dy; dz = -dz; }
+  const int z = node & (N - 1), y = (node / N) & (N - 1), x = node / (N * N);
+  return (((x + dx) & (N - 1)) * N + ((y + dy) & (N - 1))) * N + ((z + dz) & (N - 1));
+}
+
+// out[node][co] = bias[co] + sum_{tap, ci} X[nbr(node, tap)][ci] W[tap][co][ci]
+// FLIP: neighbours at -offset (the data gradient, with Wt);
+// RELU: max(., 0); MASK: times (mask[node][co] > 0) (ReLU derivative of the
+// layer below); Yb / Yf: bf16 / FP32 outputs (either may be null).
+// 256 threads = 4 wavefronts x 16 nodes; N^3 must be a multiple of 64.
+template <int CI, int CO, bool FLIP, bool RELU, bool MASK>
+__global__ void __launch_bounds__(256)
+k_conv(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __restrict__ bias,
+       const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N) {
+  static_assert(CI % 16 == 0 && CO % 16 == 0, "channels: multiples of 16");
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int node0 = blockIdx.x * 64 + wv * 16;
+  const int row = lane & 15, kq = 4 * (lane >> 4);
+  f32x4 acc[CO / 16];
+#pragma unroll
+  for (int t = 0; t < CO / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 27; ++tap) {
+    const bf16* xa = X + (size_t)nbr(node0 + row, tap, N, FLIP) * CI + kq;
+    const bf16* wb = W + ((size_t)tap * CO + row) * CI + kq;
+#pragma unroll
+    for (int cb = 0; cb < CI / 16; ++cb) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(xa + cb * 16);
+#pragma unroll
+      for (int t = 0; t < CO / 16; ++t) {
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(wb + (size_t)t * 16 * CI + cb * 16);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < CO / 16; ++t) {
+    const int co = t * 16 + row;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const size_t o = (size_t)(node0 + kq + i) * CO + co;
+      float v = acc[t][i] + bv;
+      if (RELU) v = v > 0.f ? v : 0.f;
+      if (MASK) v = b2f(mask[o]) > 0.f ? v : 0.f;
+      if (Yb) Yb[o] = f2b(v);
+      if (Yf) Yf[o] = v;
+    }
+  }
+}
+
+// Weight and bias gradients of one layer, per block over `chunk` nodes:
+//   part[blk][(tap CO + co) CI + ci] = sum_node dZ[node][co] X[nbr(node, tap)][ci]
+//   part[blk][27 CO CI + co]         = sum_node dZ[node][co]
+// (M = co, N = ci, K = nodes).  The dZ rows of 64 nodes are staged in LDS for
+// the block, the X neighbour rows per wavefront (wavefront w does taps
+// w, w + 4, ...).  chunk is a multiple of 64.
+template <int CI, int CO>
+__global__ void __launch_bounds__(256)
+k_wgrad(const bf16* __restrict__ X, const bf16* __restrict__ dZ, float* __restrict__ part, int N, int chunk) {
+  constexpr int NT = 7;                       // taps per wavefront (27 = 7 + 7 + 7 + 6)
+  __shared__ bf16 sdz[64 * CO];
+  __shared__ bf16 sx[4][64 * CI];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = lane & 15, kq = 4 * (lane >> 4);
+  f32x4 acc[NT][CO / 16][CI / 16];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int a = 0; a < CO / 16; ++a)
+#pragma unroll
+      for (int b = 0; b < CI / 16; ++b) acc[j][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float db = 0.f;
+  const size_t nbase = (size_t)blockIdx.x * chunk;
+  for (int s = 0; s < chunk; s += 64) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * CO; e += 256) sdz[e] = dZ[(nbase + s) * CO + e];
+    __syncthreads();
+    if (threadIdx.x < CO) {
+      for (int n = 0; n < 64; ++n) db += b2f(sdz[n * CO + threadIdx.x]);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int tap = wv + 4 * j;
+      if (tap >= 27) break;
+      // this lane's node: neighbour row of X
+      const int node = (int)(nbase + s) + lane;
+      const bf16* src = X + (size_t)nbr(node, tap, N, false) * CI;
+#pragma unroll
+      for (int c = 0; c < CI; c += 4)
+        *reinterpret_cast<bf16x4*>(&sx[wv][lane * CI + c]) = *reinterpret_cast<const bf16x4*>(src + c);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+        for (int a = 0; a < CO / 16; ++a) {
+          bf16x4 fa;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) fa[i] = (short)sdz[(kk * 16 + kq + i) * CO + a * 16 + row];
+#pragma unroll
+          for (int b = 0; b < CI / 16; ++b) {
+            bf16x4 fb;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) fb[i] = (short)sx[wv][(kk * 16 + kq + i) * CI + b * 16 + row];
+            acc[j][a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa, fb, acc[j][a][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  float* out = part + (size_t)blockIdx.x * (27 * CO * CI + CO);
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int tap = wv + 4 * j;
+    if (tap >= 27) break;
+#pragma unroll
+    for (int a = 0; a < CO / 16; ++a)
+#pragma unroll
+      for (int b = 0; b < CI / 16; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          out[((size_t)tap * CO + a * 16 + kq + i) * CI + b * 16 + row] = acc[j][a][b][i];
+  }
+  if (threadIdx.x < CO) out[27 * CO * CI + threadIdx.x] = db;
+}
+
+// out[i] = sum over blocks of part[b][i] (fixed order), i < n.
+static __global__ void __launch_bounds__(256)
+k_reduce_parts(const float* __restrict__ part, int nblk, int n, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int b = 0; b < nblk; ++b) s += part[(size_t)b * n + i];
+  out[i] = s;
+}
+
+// x[node][cin] FP32 -> A[node][CP] bf16 (channels cin..CP-1 zero).
+static __global__ void k_pack_input(const float* __restrict__ x, int cin, int CP, size_t nodes, bf16* __restrict__ A) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nodes * CP; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t node = e / CP;
+    const int c = (int)(e % CP);
+    A[e] = c < cin ? f2b(x[node * cin + c]) : (bf16)0;
+  }
+}
+
+// Master FP32 weights w[tap][co][ci] -> bf16 W[tap][co][ci] and Wt[tap][ci][co].
+static __global__ void k_pack_weights(const float* __restrict__ w, int CI, int CO, bf16* __restrict__ Wb,
+                                      bf16* __restrict__ Wt) {
+  const int n = 27 * CO * CI;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int tap = e / (CO * CI), co = (e / CI) % CO, ci = e % CI;
+    const bf16 b = f2b(w[e]);
+    Wb[e] = b;
+    Wt[((size_t)tap * CI + ci) * CO + co] = b;
+  }
+}
+
+// Mean squared error over nodes x cout: dY[node][co] = 2 (y - t) / (nodes cout)
+// for co < cout (0 for padding channels), per-block partial sums of (y - t)^2.
+static __global__ void __launch_bounds__(256)
+k_mse(const float* __restrict__ y, const float* __restrict__ t, int cout, int CP, size_t nodes,
+      bf16* __restrict__ dY, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  const double scale = 2.0 / ((double)nodes * cout);
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nodes * CP; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t node = e / CP;
+    const int c = (int)(e % CP);
+    float g = 0.f;
+    if (c < cout) {
+      const double d = (double)y[e] - (double)t[node * cout + c];
+      s += d * d;
+      g = (float)(scale * d);
+    }
+    dY[e] = f2b(g);
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Adam on the FP32 master parameters (Kingma & Ba 2015), step t >= 1.
+static __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                              float* __restrict__ v, int n, float lr, float b1, float b2, float eps, int t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / (1.f - powf(b1, (float)t)), vh = vi / (1.f - powf(b2, (float)t));
+  w[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// Output rows [node][CP] FP32 -> [node][cout].
+static __global__ void k_unpack_output(const float* __restrict__ yp, int CP, int cout, size_t nodes, float* __restrict__ y) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nodes * cout; e += (size_t)gridDim.x * blockDim.x)
+    y[e] = yp[(e / cout) * CP + e % cout];
+}
+
+}  // namespace cnn
+}  // namespace ld
