@@ -124,6 +124,44 @@ def time_ensemble(cfg, nreal, steps, warmup, graph=True):
     return ms, dt, bool(blown.any())
 
 
+def time_cnn(N, channels, steps):
+    """ms per forward pass and per training step (forward, backward, Adam) of
+    the learned subgrid CNN on an N^3 grid, and the MFMA FLOPs they execute
+    (padded channel counts)."""
+    import numpy as np
+    import torch
+    from lucky_droplets_amd.cnn import SubgridCNN
+    dev = torch.device("cuda:0")
+    net = SubgridCNN(N, channels)
+    net.init_params(seed=0)
+    rng = np.random.default_rng(0)
+    x = torch.from_numpy(rng.normal(size=(N ** 3, channels[0])).astype(np.float32)).to(dev)
+    t = torch.from_numpy(rng.normal(size=(N ** 3, channels[-1])).astype(np.float32)).to(dev)
+    for _ in range(2):
+        net.train_step(x, t)
+        net.forward(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        net.forward(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fwd = e0.elapsed_time(e1) / steps
+    e0.record()
+    for _ in range(steps):
+        net.train_step(x, t)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_train = e0.elapsed_time(e1) / steps
+    pad = [16 if c <= 16 else 32 for c in channels]
+    f_layer = [2 * 27 * ci * co * N ** 3 for ci, co in zip(pad[:-1], pad[1:])]
+    flops_fwd = sum(f_layer)
+    flops_train = 2 * flops_fwd + sum(f_layer[1:])        # forward + weight gradients + data gradients
+    net.close()
+    return ms_fwd, ms_train, flops_fwd, flops_train
+
+
 def time_plain(cfg, uk, n, coord, dt, steps):
     """ms per step of the unfused reference-dataflow path (ld_advance_plain)."""
     import torch
@@ -192,6 +230,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64 x 64^3 ensemble timing")
+    ap.add_argument("--no-cnn", action="store_true", help="skip the learned-subgrid CNN timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a "
@@ -282,6 +321,16 @@ def main():
                            "bytes_per_ensemble_step": be, "achieved_GBs": be / (mse * 1e-3) / 1e9,
                            "frac": be / (mse * 1e-3) / 1e9 / HBM_PEAK_GBS, "blowup": blown_e}
         log(f"[bench] ensemble {mse:.3f} ms per ensemble step")
+    if not args.no_cnn:
+        ch = (3, 32, 32, 32, 3)
+        log("[bench] learned subgrid CNN: 128^3, channels " + str(ch))
+        msf, mst, ff, ft = time_cnn(128, ch, max(5, args.steps // 2))
+        out["cnn"] = {"workload": "4-layer periodic 3x3x3 CNN, channels 3-32-32-32-3 (padded to 16/32), 128^3 nodes, "
+                                  "bf16 MFMA with FP32 accumulation",
+                      "ms_forward": msf, "ms_train_step": mst,
+                      "forward_TFLOPs": ff / (msf * 1e-3) / 1e12, "train_TFLOPs": ft / (mst * 1e-3) / 1e12,
+                      "peak_TFLOPs": 2500.0, "frac_forward": ff / (msf * 1e-3) / 1e12 / 2500.0}
+        log(f"[bench] CNN forward {msf:.3f} ms, train step {mst:.3f} ms")
     out["blowup"] = bool(blown)
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, bounded sample)")
