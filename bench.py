@@ -52,14 +52,16 @@ def make_inputs(cfg):
     return uk, n, coord, pid, dt
 
 
-def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None, graph=False):
+def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None, graph=False, autotune=False):
     """ms per step over `steps` timed steps (after `warmup`), plus the summed
     HIP-event duration and launch count of `profile_kernel` inside the timed region
-    (profiling runs the steps eagerly; graph=True replays a captured HIP graph)."""
+    (profiling runs the steps eagerly; graph=True replays a captured HIP graph;
+    autotune=True lets the context pick the y-pass block shape at creation)."""
     import torch
     from lucky_droplets_amd.dns import ForcedDNS
     dev = torch.device("cuda:0")
-    d = ForcedDNS(cfg, graph=graph)
+    d = ForcedDNS(cfg, graph=graph, autotune=autotune)
+    tune = d.tuning() if autotune else None
     ukg = torch.from_numpy(uk).to(dev).to(d.cdtype)
     ng = torch.from_numpy(n).to(dev).to(d.rdtype)
     cg = torch.from_numpy(coord).to(dev)
@@ -82,6 +84,7 @@ def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None, graph=Fa
         d.profile(profile_kernel, False)
     lastmax, blown = d.status()
     d.close()
+    time_gpu.last_tuning = tune
     return ms, kern, lastmax, blown
 
 
@@ -231,6 +234,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64 x 64^3 ensemble timing")
     ap.add_argument("--no-cnn", action="store_true", help="skip the learned-subgrid CNN timing")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default y-pass block shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a "
@@ -245,13 +249,17 @@ def main():
     log(f"[bench] inputs ready in {time.time() - t:.1f}s, dt={dt:.4e}")
 
     log(f"[bench] FP32 timing: warmup {args.warmup}, steps {args.steps}")
+    tuned = not args.no_autotune
     ms32, (kms, klaunch), lastmax, blown = time_gpu(cfg, uk, n, coord, dt, args.steps, args.warmup,
-                                                    profile_kernel="droplets" if cfg.n_particles else None)
-    log(f"[bench] FP32 {ms32:.3f} ms/step; droplet kernel {kms / max(klaunch, 1):.3f} ms x {klaunch}")
+                                                    profile_kernel="droplets" if cfg.n_particles else None,
+                                                    autotune=tuned)
+    tuning = time_gpu.last_tuning
+    log(f"[bench] FP32 {ms32:.3f} ms/step; droplet kernel {kms / max(klaunch, 1):.3f} ms x {klaunch}"
+        + (f"; y-pass lines per block {tuning[0]} (us {tuning[1]})" if tuning else ""))
     ms_eager = ms32
     msg = None
     if not args.no_graph:
-        msg, _, _, blown_g = time_gpu(cfg, uk, n, coord, dt, args.steps, args.warmup, graph=True)
+        msg, _, _, blown_g = time_gpu(cfg, uk, n, coord, dt, args.steps, args.warmup, graph=True, autotune=tuned)
         log(f"[bench] FP32 with HIP-graph replay {msg:.3f} ms/step")
         blown = blown or blown_g
         ms32 = min(ms32, msg)
@@ -266,6 +274,7 @@ def main():
         "ms_per_step": ms32,
         "eager_ms_per_step": ms_eager,
         "graph_ms_per_step": msg,
+        "y_pass_tuning": ({"lines_per_block": tuning[0], "median_us_M_Mhalf_2M": tuning[1]} if tuning else None),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

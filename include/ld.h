@@ -135,9 +135,10 @@ int ld_status(ld_ctx* ctx, double* last_max, int32_t* blown_up);
  * entries.  No counterpart in the reference (one realization per job). */
 int ld_status_all(ld_ctx* ctx, double* last_max, int32_t* blown_up);
 /* The autotuner's choice: lines per block of the y passes (*ylines, host) and
- * the measured median times in microseconds of the two candidates (us[2],
- * host: M and M/2 lines; zeros when the context was created without
- * autotune).  No counterpart in the reference. */
+ * the measured median times in microseconds of the candidates (us[3], host:
+ * M, M/2 and 2M lines with M = 16 for FP32 and 8 for FP64; 0 for a candidate
+ * not timed, all zeros when the context was created without autotune).  No
+ * counterpart in the reference. */
 int ld_tuning(ld_ctx* ctx, int32_t* ylines, double* us);
 
 /* ---------------------------------------------------------- measurement */

@@ -166,16 +166,19 @@ struct Engine final : EngineBase {
   }
   int batch() const override { return B; }
 
-  // Autotuner: time the 6-field y-inverse pass with M and M/2 lines per block
-  // (after one warm-up launch each, the median of five) and keep the faster.
-  double tune_us[2] = {0.0, 0.0};
+  // Autotuner: time the 6-field y-inverse pass with M, M/2 and 2M lines per
+  // block (after one warm-up launch each, the median of five) and keep the
+  // fastest.
+  double tune_us[3] = {0.0, 0.0, 0.0};
   int autotune() {
     LD_TRY(fill<C<T>>(st, w6, 6 * CF, C<T>{}));
     hipEvent_t e0, e1;
     LD_HIP(hipEventCreate(&e0));
     LD_HIP(hipEventCreate(&e1));
-    const int cand[2] = {M, MH};
-    for (int c = 0; c < 2; ++c) {
+    const int cand[3] = {M, MH, M2};
+    const bool wide_ok = (N / nslab) % M2 == 0 && M2 * N1 <= 1024;
+    for (int c = 0; c < 3; ++c) {
+      if (c == 2 && !wide_ok) { tune_us[2] = 0.0; continue; }
       my = cand[c];
       std::vector<float> t;
       for (int r = 0; r < 6; ++r) {
@@ -192,12 +195,15 @@ struct Engine final : EngineBase {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    my = tune_us[1] < tune_us[0] ? MH : M;
+    int best = 0;
+    for (int c = 1; c < 3; ++c)
+      if (tune_us[c] > 0.0 && tune_us[c] < tune_us[best]) best = c;
+    my = cand[best];
     return LD_OK;
   }
   int tuning(int32_t* ylines, double* us) override {
     if (ylines) *ylines = my;
-    if (us) { us[0] = tune_us[0]; us[1] = tune_us[1]; }
+    if (us) { us[0] = tune_us[0]; us[1] = tune_us[1]; us[2] = tune_us[2]; }
     return LD_OK;
   }
 
@@ -288,9 +294,10 @@ struct Engine final : EngineBase {
 
   // ------------------------------------------------------------ launch helpers
   // Line pass over x-planes [x0, x0 + nx) (AXIS 1) or all lines (AXIS 0).
-  // y passes run with MY lines per block: M, or M/2 when the autotuner
-  // measured that faster (the lines a block owns do not change the arithmetic).
-  static constexpr int MH = M / 2;
+  // y passes run with my lines per block: M, M/2 or 2M (2M needs N/nslab to be
+  // a multiple of 2M), the autotuner's choice; the lines a block owns do not
+  // change the arithmetic.
+  static constexpr int MH = M / 2, M2 = 2 * M;
   int my = M;
   template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale, int x0 = 0,
                                         int nx = N) {
@@ -299,6 +306,9 @@ struct Engine final : EngineBase {
     LD_TRY(prof_mark(kid, s));
     if (AXIS == 1 && my == MH)
       hipLaunchKernelGGL((k_pass<T, N, MH, AXIS, DIR>), dim3(nx * Nf / MH, nf, B), dim3(MH * N1), 0, s, in, out, CF,
+                         (const C<T>*)tw, scale, qoff, rs);
+    else if (AXIS == 1 && my == M2)
+      hipLaunchKernelGGL((k_pass<T, N, M2, AXIS, DIR>), dim3(nx * Nf / M2, nf, B), dim3(M2 * N1), 0, s, in, out, CF,
                          (const C<T>*)tw, scale, qoff, rs);
     else
       hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(AXIS == 1 ? nx * Nf / M : NBLK_X, nf, B), dim3(M * N1), 0, s,

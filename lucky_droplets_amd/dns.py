@@ -254,11 +254,11 @@ class ForcedDNS:
         _lib.check(self.lib.ld_advance(self.ctx, float(h), int(nsteps)))
 
     def tuning(self):
-        """(y-pass lines per block, [median us with M and with M/2 lines])."""
+        """(y-pass lines per block, [median us with M, M/2 and 2M lines])."""
         yl = ctypes.c_int32()
-        us = (ctypes.c_double * 2)()
+        us = (ctypes.c_double * 3)()
         _lib.check(self.lib.ld_tuning(self.ctx, ctypes.byref(yl), us))
-        return yl.value, [us[0], us[1]]
+        return yl.value, [us[0], us[1], us[2]]
 
     def status_all(self):
         """(last max Re(u_hat), blown-up flag) per realization, as numpy arrays."""

@@ -456,7 +456,7 @@ def test_autotune_keeps_results(gpu):
         outs.append([x.cpu().numpy() for x in t])
         if tune:
             yl, us = d.tuning()
-            assert yl in (16, 8) and min(us) > 0
+            assert yl in (16, 8, 32) and min(us[:2]) > 0
         d.close()
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
