@@ -244,25 +244,46 @@ k_particles_stage(PStage<T, PT> A0) {
   const int pp = live ? p : 0;
   constexpr size_t R = (size_t)N * N * N;
   const int cap = A.cap;
-  double xs[3], vp[3], m;
+  // Loads are grouped into three round trips (stage position; the gather;
+  // velocity, mass and the sump inputs), with uniform pointer selects rather
+  // than branches between a load and its use: stage 0 reads x0 + 0 * x0 = x0
+  // (exact for finite values).  The stage velocity and mass are read after the
+  // gather, so they hold no registers while its loads are in flight.
+  const bool s0 = A.stage == 0;
+  const double ah = s0 ? 0.0 : A.ah;
+  const double* kxp = s0 ? A.x0 : A.kx;
+  double x0v[3], xs[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double x = A.x0[k * cap + pp];
-    double vv = (double)A.v0[k * cap + pp];
-    if (A.stage > 0) {
-      x = x + A.ah * A.kx[k * cap + pp];
-      vv = (double)(PT)(A.v0[k * cap + pp] + (PT)A.ah * A.kv[k * cap + pp]);
-    }
-    xs[k] = x; vp[k] = vv;
-  }
-  m = (double)A.v0[3 * cap + pp];
-  if (A.stage > 0) m = (double)(PT)(A.v0[3 * cap + pp] + (PT)A.ah * A.kv[3 * cap + pp]);
+  for (int k = 0; k < 3; ++k) x0v[k] = A.x0[k * cap + pp];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) xs[k] = x0v[k] + ah * kxp[k * cap + pp];
   const double L = A.P.L, dx = A.P.dx, pi_2dx = A.P.pi_2dx;
   const double xw = wrapL(xs[0], L), yw = wrapL(xs[1], L), zw = wrapL(xs[2], L);
   // one sincos per axis (cosW4); node i0 + a has weight (1 +- {cos, sin})/4.
   // The angles and the sincos are FP64; the weights and the gather sums are
   // kept in the field precision GT (FP32 fields: FP32 sums of 64 terms).
   using GT = std::conditional_t<sizeof(T) == 4, float, double>;
+  // deposit input for the brick pass (k_brick_deposit), written before the
+  // gather so the wrapped position holds no registers across it: the wrapped
+  // stage position now, W after the RHS.  Slots whose stage bin left the
+  // +-1-bin window around their sorted bin go to the fallback list.
+  bool far = false;
+  if (live && c == 0) {
+    far = A.all_fallback != 0;
+    const double xb[3] = {xw, yw, zw};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      // bins exactly as k_bin_count forms them
+      int cs = (int)(xb[k] / dx), c0 = (int)(wrapL(x0v[k], L) / dx);
+      cs = cs < 0 ? 0 : (cs > N - 1 ? N - 1 : cs);
+      c0 = c0 < 0 ? 0 : (c0 > N - 1 ? N - 1 : c0);
+      const int d = ((cs / Bins<N>::BS - c0 / Bins<N>::BS) % Bins<N>::B + Bins<N>::B) % Bins<N>::B;
+      far = far || (d > 1 && d < Bins<N>::B - 1);
+    }
+    double* dp = reinterpret_cast<double*>(A.dep + p);
+    dp[0] = xw; dp[1] = yw; dp[2] = zw;
+    if (far) A.flist[atomicAdd(A.fcount, 1)] = p;
+  }
   const int i0 = (int)floor(xw / dx) - 1, k0 = (int)floor(zw / dx) - 1;
   double xsn, xcs, zsn, zcs;
   sincos((xw - (double)i0 * dx) * pi_2dx, &xsn, &xcs);
@@ -282,30 +303,58 @@ k_particles_stage(PStage<T, PT> A0) {
   GT Ig[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) Ig[k] = (GT)0;
-  if (live) {
+  // FP32 fields: all 32 record loads are written ahead of the sums, so the
+  // compiler keeps about a dozen in flight per lane (at 5 waves per SIMD);
+  // this measured faster than a plane-by-plane loop at 8 waves per SIMD.
+  // FP64 records are twice the size; they keep the loop.
+  if (sizeof(T) == 4 && live) {
+    V4<T> U[16], V[16];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const uint32_t plane = (uint32_t)wrapIdx<N>(i0 + a) * N;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t o = (plane + (uint32_t)iy[b]) * N + (uint32_t)iz;
+        U[a * 4 + b] = ld32(A.gu, o);
+        V[a * 4 + b] = ld32(A.gv, o);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const GT wx = (GT)0.25 * ((GT)1 + ((a & 2) ? (GT)-1 : (GT)1) * ((a & 1) ? gxs : gxc));
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const GT w = wy[b] * wx;
+        const V4<T>& u = U[a * 4 + b];
+        const V4<T>& v = V[a * 4 + b];
+        Ig[0] += (GT)u.x * w;
+        Ig[1] += (GT)u.y * w;
+        Ig[2] += (GT)u.z * w;
+        Ig[3] += (GT)v.x * w;
+        Ig[4] += (GT)v.y * w;
+        Ig[5] += (GT)v.z * w;
+        Ig[6] += (GT)v.w * w;
+      }
+    }
+  } else if (live) {
 #pragma unroll 1
     for (int a = 0; a < 4; ++a) {
       const GT wx = (GT)0.25 * ((GT)1 + ((a & 2) ? (GT)-1 : (GT)1) * ((a & 1) ? gxs : gxc));
-      const size_t plane = (size_t)wrapIdx<N>(i0 + a) * N;
-      GT t[7];
-#pragma unroll
-      for (int k = 0; k < 7; ++k) t[k] = (GT)0;
+      const uint32_t plane = (uint32_t)wrapIdx<N>(i0 + a) * N;
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const size_t o = (plane + iy[b]) * N + iz;
-        const GT w = wy[b];
-        const V4<T> U = A.gu[o];
-        const V4<T> V = A.gv[o];
-        t[0] += (GT)U.x * w;
-        t[1] += (GT)U.y * w;
-        t[2] += (GT)U.z * w;
-        t[3] += (GT)V.x * w;
-        t[4] += (GT)V.y * w;
-        t[5] += (GT)V.z * w;
-        t[6] += (GT)V.w * w;
+        const uint32_t o = (plane + (uint32_t)iy[b]) * N + (uint32_t)iz;   // < N^3 <= 2^27
+        const GT w = wy[b] * wx;
+        const V4<T> U = ld32(A.gu, o);
+        const V4<T> V = ld32(A.gv, o);
+        Ig[0] += (GT)U.x * w;
+        Ig[1] += (GT)U.y * w;
+        Ig[2] += (GT)U.z * w;
+        Ig[3] += (GT)V.x * w;
+        Ig[4] += (GT)V.y * w;
+        Ig[5] += (GT)V.z * w;
+        Ig[6] += (GT)V.w * w;
       }
-#pragma unroll
-      for (int k = 0; k < 7; ++k) Ig[k] += t[k] * wx;
     }
   }
   double I[7];
@@ -316,6 +365,21 @@ k_particles_stage(PStage<T, PT> A0) {
     v += __shfl_xor(v, 2, 64);
     I[k] = (double)v;
   }
+  // stage velocity and mass (v0 + a h kv) and the sump inputs (x0, v0 at
+  // stage 0, else the accumulators)
+  const PT* kvp = s0 ? A.v0 : A.kv;
+  const double* sxp = s0 ? A.x0 : A.sx;
+  const PT* svp = s0 ? A.v0 : A.sv;
+  PT v0v[4], kvv[4], prv[4];
+  double prx[3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { v0v[k] = A.v0[k * cap + pp]; kvv[k] = kvp[k * cap + pp]; prv[k] = svp[k * cap + pp]; }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) prx[k] = sxp[k * cap + pp];
+  double vp[3], m;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) vp[k] = (double)(PT)(v0v[k] + (PT)ah * kvv[k]);
+  m = (double)(PT)(v0v[3] + (PT)ah * kvv[3]);
   double rhs[7], W = 0.0;
   if (live) dropletRhs(A.P, vp, m, I, rhs, W);
   // largest growth rate of the launch: the bound of the brick deposit's fixed-point scale
@@ -332,37 +396,15 @@ k_particles_stage(PStage<T, PT> A0) {
     }
   }
   if (!live || c != 0) return;
-  // deposit input for the brick pass (k_brick_deposit); slots whose stage
-  // bin left the +-1-bin window around their sorted bin go to the fallback list
-  {
-    bool far = A.all_fallback != 0;
-    const double xb[3] = {xw, yw, zw};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      // bins exactly as k_bin_count forms them
-      int cs = (int)(xb[k] / dx), c0 = (int)(wrapL(A.x0[k * cap + p], L) / dx);
-      cs = cs < 0 ? 0 : (cs > N - 1 ? N - 1 : cs);
-      c0 = c0 < 0 ? 0 : (c0 > N - 1 ? N - 1 : c0);
-      const int d = ((cs / Bins<N>::BS - c0 / Bins<N>::BS) % Bins<N>::B + Bins<N>::B) % Bins<N>::B;
-      far = far || (d > 1 && d < Bins<N>::B - 1);
-    }
-    // a fallback slot stores -(W + 1) so the brick pass skips it
-    A.dep[p] = make_double4(xw, yw, zw, far ? -(W + 1.0) : W);
-    if (far) A.flist[atomicAdd(A.fcount, 1)] = p;
-  }
+  // a fallback slot stores -(W + 1) so the brick pass skips it
+  reinterpret_cast<double*>(A.dep + p)[3] = far ? -(W + 1.0) : W;
   // sump (forced-dns-sm.py:337, 342-358); the last stage writes the new state
+  double* dxp = A.stage == 3 ? A.x0 : A.sx;
+  PT* dvp = A.stage == 3 ? A.v0 : A.sv;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const double prev = A.stage == 0 ? A.x0[k * cap + p] : A.sx[k * cap + p];
-    const double nv = prev + A.wh * rhs[k];
-    if (A.stage == 3) A.x0[k * cap + p] = nv; else A.sx[k * cap + p] = nv;
-  }
+  for (int k = 0; k < 3; ++k) dxp[k * cap + p] = prx[k] + A.wh * rhs[k];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const PT prev = A.stage == 0 ? A.v0[k * cap + p] : A.sv[k * cap + p];
-    const PT nv = prev + (PT)A.wh * (PT)rhs[3 + k];
-    if (A.stage == 3) A.v0[k * cap + p] = nv; else A.sv[k * cap + p] = nv;
-  }
+  for (int k = 0; k < 4; ++k) dvp[k * cap + p] = prv[k] + (PT)A.wh * (PT)rhs[3 + k];
   if (A.stage < 3) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) A.kx[k * cap + p] = rhs[k];
