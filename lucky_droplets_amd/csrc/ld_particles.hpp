@@ -40,6 +40,15 @@ __device__ __forceinline__ void cosWeights(double x, double dx, double pi_2dx, i
   for (int a = 0; a < 4; ++a) idx[a] = wrapIdx<N>(i0 + a);
 }
 
+// sin and cos of an FP64 angle in precision G (FP32: of the angle rounded to FP32).
+template <typename G> __device__ __forceinline__ void sincosGT(double th, G& s, G& c) {
+  if constexpr (sizeof(G) == 4) {
+    sincosf((float)th, &s, &c);
+  } else {
+    sincos(th, &s, &c);
+  }
+}
+
 // Periodic wrap of one coordinate, x mod L with numpy's sign convention
 // (particles.py:269 `x[:, :d] %= L`).
 __device__ __forceinline__ double wrapL(double x, double L) {
@@ -284,20 +293,20 @@ k_particles_stage(PStage<T, PT> A0) {
     dp[0] = xw; dp[1] = yw; dp[2] = zw;
     if (far) A.flist[atomicAdd(A.fcount, 1)] = p;
   }
-  const int i0 = (int)floor(xw / dx) - 1, k0 = (int)floor(zw / dx) - 1;
-  double xsn, xcs, zsn, zcs;
-  sincos((xw - (double)i0 * dx) * pi_2dx, &xsn, &xcs);
-  sincos((zw - (double)k0 * dx) * pi_2dx, &zsn, &zcs);
-  GT wy[4];
+  const int i0 = (int)floor(xw / dx) - 1, j0 = (int)floor(yw / dx) - 1, k0 = (int)floor(zw / dx) - 1;
+  // the angles are formed in FP64; their sines and cosines are taken in the
+  // weight precision GT (the FP32 form rounds the angle in [pi/2, pi) to FP32)
+  GT xsn, xcs, ysn, ycs, zsn, zcs;
+  sincosGT((xw - (double)i0 * dx) * pi_2dx, xsn, xcs);
+  sincosGT((yw - (double)j0 * dx) * pi_2dx, ysn, ycs);
+  sincosGT((zw - (double)k0 * dx) * pi_2dx, zsn, zcs);
+  const GT q = (GT)0.25;
+  const GT wy[4] = {q * ((GT)1 + ycs), q * ((GT)1 + ysn), q * ((GT)1 - ycs), q * ((GT)1 - ysn)};
   int iy[4];
-  {
-    double wyd[4];
-    const int j0 = cosW4(yw, dx, pi_2dx, wyd);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) { wy[b] = (GT)wyd[b]; iy[b] = wrapIdx<N>(j0 + b); }
-  }
-  const GT wz = (GT)(0.25 * (1.0 + ((c & 2) ? -1.0 : 1.0) * ((c & 1) ? zsn : zcs)));
-  const GT gxs = (GT)xsn, gxc = (GT)xcs;
+  for (int b = 0; b < 4; ++b) iy[b] = wrapIdx<N>(j0 + b);
+  const GT wz = q * ((GT)1 + ((c & 2) ? (GT)-1 : (GT)1) * ((c & 1) ? zsn : zcs));
+  const GT gxs = xsn, gxc = xcs;
   const int iz = wrapIdx<N>(k0 + c);
   // gather: this lane's z node, all (x, y) nodes; two 16-byte records per node
   GT Ig[7];
