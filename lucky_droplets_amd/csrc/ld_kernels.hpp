@@ -159,6 +159,10 @@ struct ZTile {
   __device__ __forceinline__ C<T>& spec(int f, int p, int k) const {
     return region(f * (P / 2) + p / 2)[(p & 1) * Nf + k];
   }
+  // four consecutive real entries j..j+3 (j % 4 == 0; regions are 32-byte aligned)
+  __device__ __forceinline__ V4<T>& re4(int f, int p, int j) const {
+    return *reinterpret_cast<V4<T>*>(&re(f, p, j));
+  }
 };
 
 // Inverse z transform (c2r, unnormalised) of FIN fields already loaded as
@@ -292,6 +296,20 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
 }
 
 // ------------------------------------------------------------- z-pass ops
+// The physical-space work of an op visits its P lines (x F fields) with the
+// block's compile-time thread count NT: item e of thread t is t + it * NT,
+// it < IT, so the loop is unrolled and all of a thread's global loads are
+// issued before any is used.  An item is a quad of 4 consecutive z entries
+// (planar arrays: 16-byte FP32 accesses) or one entry (node-interleaved
+// records, which are 16 bytes already).
+template <int NT, int NQ> struct Quads {
+  static constexpr int IT = (NQ + NT - 1) / NT;
+  static __device__ __forceinline__ int e(int it) { return (int)threadIdx.x + it * NT; }
+  static __device__ __forceinline__ bool ok(int it) { return NQ % NT == 0 || e(it) < NQ; }
+};
+template <typename T> __device__ __forceinline__ V4<T> scale4(const V4<T>& a, T s) {
+  V4<T> r; r.x = a.x * s; r.y = a.y * s; r.z = a.z * s; r.w = a.w * s; return r;
+}
 // Plain c2r along z of F fields, times `scale` (last pass of irfft_mpi,
 // forced-dns-sm.py:212).
 template <typename T, int N, int P, int F>
@@ -301,9 +319,12 @@ struct OpZInv {
   __device__ const C<T>* specIn(int f) const { return in + f * fin; }
   __device__ C<T>* specOut(int) const { return nullptr; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
-    for (int e = threadIdx.x; e < F * P * N; e += blockDim.x) {
-      const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
-      out[f * fout + (size_t)(line0 + p) * N + j] = t.re(f, p, j) * scale;
+    using Q = Quads<ZThreads<N, P, F, 0>::v, F * P * N / 4>;
+#pragma unroll
+    for (int it = 0; it < Q::IT; ++it) {
+      if (!Q::ok(it)) break;
+      const int e4 = 4 * Q::e(it), f = e4 / (P * N), rem = e4 % (P * N), p = rem / N, j = rem % N;
+      *reinterpret_cast<V4<T>*>(out + f * fout + (size_t)(line0 + p) * N + j) = scale4(t.re4(f, p, j), scale);
     }
   }
 };
@@ -315,9 +336,19 @@ struct OpZFwd {
   __device__ const C<T>* specIn(int) const { return nullptr; }
   __device__ C<T>* specOut(int f) const { return out + f * fout; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
-    for (int e = threadIdx.x; e < F * P * N; e += blockDim.x) {
-      const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
-      t.re(f, p, j) = in[f * fin + (size_t)(line0 + p) * N + j];
+    using Q = Quads<ZThreads<N, P, 0, F>::v, F * P * N / 4>;
+    V4<T> q[Q::IT];
+#pragma unroll
+    for (int it = 0; it < Q::IT; ++it) {
+      if (!Q::ok(it)) break;
+      const int e4 = 4 * Q::e(it), f = e4 / (P * N), rem = e4 % (P * N), p = rem / N, j = rem % N;
+      q[it] = *reinterpret_cast<const V4<T>*>(in + f * fin + (size_t)(line0 + p) * N + j);
+    }
+#pragma unroll
+    for (int it = 0; it < Q::IT; ++it) {
+      if (!Q::ok(it)) break;
+      const int e4 = 4 * Q::e(it), f = e4 / (P * N), rem = e4 % (P * N), p = rem / N, j = rem % N;
+      t.re4(f, p, j) = q[it];
     }
   }
 };
@@ -330,9 +361,12 @@ struct OpZRound {
   __device__ const C<T>* specIn(int f) const { return buf + f * fs; }
   __device__ C<T>* specOut(int f) const { return buf + f * fs; }
   __device__ void physical(const ZTile<T, N, P>& t, int) const {
-    for (int e = threadIdx.x; e < 3 * P * N; e += blockDim.x) {
-      const int f = e / (P * N), rem = e % (P * N), p = rem / N, j = rem % N;
-      t.re(f, p, j) *= scale;
+    using Q = Quads<ZThreads<N, P, 3, 3>::v, 3 * P * N / 4>;
+#pragma unroll
+    for (int it = 0; it < Q::IT; ++it) {
+      if (!Q::ok(it)) break;
+      const int e4 = 4 * Q::e(it), f = e4 / (P * N), rem = e4 % (P * N), p = rem / N, j = rem % N;
+      t.re4(f, p, j) = scale4(t.re4(f, p, j), scale);
     }
   }
 };
@@ -352,8 +386,13 @@ struct OpZCross {
   __device__ C<T>* specOut(int f) const { return w + f * fs; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
-    for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
-      const int p = e / N, j = e % N;
+    // one element per thread and step, so consecutive lanes write
+    // consecutive 16-byte records
+    using Q = Quads<ZThreads<N, P, 6, 3>::v, P * N>;
+#pragma unroll
+    for (int it = 0; it < Q::IT; ++it) {
+      if (!Q::ok(it)) break;
+      const int e = Q::e(it), p = e / N, j = e % N;
       const T ux = t.re(0, p, j) * scale, uy = t.re(1, p, j) * scale, uz = t.re(2, p, j) * scale;
       const T wx = t.re(3, p, j) * scale, wy = t.re(4, p, j) * scale, wz = t.re(5, p, j) * scale;
       const size_t o = (size_t)(line0 + p) * N + j;
@@ -394,12 +433,23 @@ struct OpZVel {
   __device__ C<T>* specOut(int f) const { return wout + f * fs; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     constexpr size_t R = (size_t)N * N * N;
-    for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
-      const int p = e / N, j = e % N;
+    // one element per thread and step (consecutive lanes write consecutive
+    // 16-byte records); all loads of n are issued before the first use
+    using Q = Quads<ZThreads<N, P, 3, 3>::v, P * N>;
+    T nq[Q::IT];
+#pragma unroll
+    for (int it = 0; it < Q::IT; ++it) {
+      if (!Q::ok(it)) break;
+      nq[it] = n[(size_t)line0 * N + Q::e(it)];
+    }
+#pragma unroll
+    for (int it = 0; it < Q::IT; ++it) {
+      if (!Q::ok(it)) break;
+      const int e = Q::e(it), p = e / N, j = e % N;
       const size_t o = (size_t)(line0 + p) * N + j;
       const T vx = t.re(0, p, j) * scale, vy = t.re(1, p, j) * scale;
       const T vz = t.re(2, p, j) * scale - tpsg;
-      const T nn = nStage<T>(n[o], clip);
+      const T nn = nStage<T>(nq[it], clip);
       if (v) { v[o] = vx; v[R + o] = vy; v[2 * R + o] = vz; }
       if (gv) { V4<T> q; q.x = vx; q.y = vy; q.z = vz; q.w = nn; gv[o] = q; }
       t.re(0, p, j) = nn * vx;
@@ -433,21 +483,42 @@ struct OpZKn {
   __device__ C<T>* specOut(int) const { return nullptr; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     double s_old = 0.0, s_new = 0.0;
-    for (int e = threadIdx.x; e < P * N; e += blockDim.x) {
-      const int p = e / N, j = e % N;
-      const size_t o = (size_t)(line0 + p) * N + j;
-      const T kn = t.re(0, p, j) * scale;
-      if (kn_out) kn_out[o] = kn;
+    using Q = Quads<ZThreads<N, P, 1, 0>::v, P * N / 4>;
+    const size_t o0 = (size_t)line0 * N;
+    V4<T> qb[Q::IT], qa[Q::IT];
+    if (nacc) {
+      const T* accsrc = acc_init ? n0 : nacc;
+#pragma unroll
+      for (int it = 0; it < Q::IT; ++it) {
+        if (!Q::ok(it)) break;
+        const size_t o = o0 + 4 * Q::e(it);
+        qb[it] = *reinterpret_cast<const V4<T>*>(n0 + o);
+        qa[it] = *reinterpret_cast<const V4<T>*>(accsrc + o);
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < Q::IT; ++it) {
+      if (!Q::ok(it)) break;
+      const int e4 = 4 * Q::e(it), p = e4 / N, j = e4 % N;
+      const size_t o = o0 + e4;
+      const V4<T> kn = scale4(t.re4(0, p, j), scale);
+      if (kn_out) *reinterpret_cast<V4<T>*>(kn_out + o) = kn;
       if (nacc) {
-        const T base = n0[o];
-        const T a = (acc_init ? base : nacc[o]) + wacc * kn;
-        nacc[o] = a;
-        if (sum_target == 1) { s_old += (double)a; s_new += a > (T)0 ? (double)a : 0.0; }
-        if (nnext) {
-          const T b = (base > (T)0 ? base : (T)0) + kn * anext;
-          nnext[o] = b;
-          if (sum_target == 0) { s_old += (double)b; s_new += b > (T)0 ? (double)b : 0.0; }
+        const T kv[4] = {kn.x, kn.y, kn.z, kn.w};
+        const T bv[4] = {qb[it].x, qb[it].y, qb[it].z, qb[it].w};
+        const T av[4] = {qa[it].x, qa[it].y, qa[it].z, qa[it].w};
+        T ao[4], bo[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const T a = av[i] + wacc * kv[i];
+          ao[i] = a;
+          if (sum_target == 1) { s_old += (double)a; s_new += a > (T)0 ? (double)a : 0.0; }
+          const T b = (bv[i] > (T)0 ? bv[i] : (T)0) + kv[i] * anext;
+          bo[i] = b;
+          if (nnext && sum_target == 0) { s_old += (double)b; s_new += b > (T)0 ? (double)b : 0.0; }
         }
+        *reinterpret_cast<V4<T>*>(nacc + o) = V4<T>{ao[0], ao[1], ao[2], ao[3]};
+        if (nnext) *reinterpret_cast<V4<T>*>(nnext + o) = V4<T>{bo[0], bo[1], bo[2], bo[3]};
       }
     }
     if (sum_target >= 0) {
