@@ -12,6 +12,9 @@
  *
  * Conventions
  *  - All buffer arguments are DEVICE pointers (e.g. torch tensor.data_ptr()).
+ *    Field arrays (physical and spectral) must be 32-byte aligned (they are
+ *    moved as 16/32-byte vectors; torch allocations are 256-byte aligned);
+ *    a misaligned field pointer returns LD_ERR_ARG.
  *  - Field precision follows the context: LD_F32 -> float / complex64,
  *    LD_F64 -> double / complex128.  Droplet rows are always float64.
  *  - Layouts are the reference's at one MPI rank (C order):

@@ -37,6 +37,17 @@ using ld::set_error;
     }                                                   \
   } while (0)
 
+// Field arrays (real and spectral) are moved as 16-byte (FP32) or 32-byte
+// (FP64) vectors: a non-null field pointer must be 32-byte aligned.
+#define LD_ALIGNED(ptr)                                 \
+  do {                                                  \
+    if ((uintptr_t)(ptr) % 32 != 0) {                   \
+      set_error("%s: field array " #ptr " must be 32-byte aligned", __func__); \
+      return LD_ERR_ARG;                                \
+    }                                                   \
+  } while (0)
+#define LD_FIELD(ptr) do { LD_NONNULL(ptr); LD_ALIGNED(ptr); } while (0)
+
 // Entry points that work on one realization reject an ensemble context.
 #define LD_ONE(c)                                                                        \
   do {                                                                                   \
@@ -115,30 +126,31 @@ int ld_set_stream(ld_ctx* ctx, void* s) {
 }
 
 int ld_rfft(ld_ctx* ctx, const void* u, void* fu, int nfields) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(u); LD_NONNULL(fu);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(u); LD_FIELD(fu);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->rfft(u, fu, nfields);
 }
 
 int ld_irfft(ld_ctx* ctx, const void* fu, void* u, int nfields) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(fu); LD_NONNULL(u);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(fu); LD_FIELD(u);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->irfft(fu, u, nfields);
 }
 
 int ld_forcing(ld_ctx* ctx, const void* uk, void* fk, double* ek_out) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(fk);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(uk); LD_FIELD(fk);
   return ctx->eng->forcing(uk, fk, ek_out);
 }
 
 int ld_clip_zero(ld_ctx* ctx, void* x, void* out) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(x); LD_NONNULL(out);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(x); LD_FIELD(out);
   return ctx->eng->clip_zero(x, out);
 }
 
 int ld_full_rhs(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np, void* ku, void* kn,
                 double* kp, void* fc, double visc, double forc, void* u_out, void* v_out) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(n); LD_NONNULL(ku); LD_NONNULL(kn); LD_NONNULL(fc);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(uk); LD_FIELD(n); LD_FIELD(ku); LD_FIELD(kn); LD_FIELD(fc);
+  LD_ALIGNED(u_out); LD_ALIGNED(v_out);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(kp); }
   return ctx->eng->full_rhs(uk, n, coord, np, ku, kn, kp, fc, visc, forc, u_out, v_out);
@@ -146,21 +158,21 @@ int ld_full_rhs(ld_ctx* ctx, const void* uk, const void* n, const double* coord,
 
 int ld_rk4(ld_ctx* ctx, double h, const void* uk, void* n, const double* coord, int64_t np, void* uknew, void* nnew,
            double* coord_new) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(n); LD_NONNULL(uknew); LD_NONNULL(nnew);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(uk); LD_FIELD(n); LD_FIELD(uknew); LD_FIELD(nnew);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(coord_new); }
   return ctx->eng->rk4(h, uk, n, coord, np, uknew, nnew, coord_new);
 }
 
 int ld_state_set(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np) {
-  LD_CHECK_CTX(ctx); LD_NONNULL(uk); LD_NONNULL(n);
+  LD_CHECK_CTX(ctx); LD_FIELD(uk); LD_FIELD(n);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) LD_NONNULL(coord);
   return ctx->eng->state_set(uk, n, coord, np);
 }
 
 int ld_state_get(ld_ctx* ctx, void* uk, void* n, double* coord) {
-  LD_CHECK_CTX(ctx);
+  LD_CHECK_CTX(ctx); LD_ALIGNED(uk); LD_ALIGNED(n);
   return ctx->eng->state_get(uk, n, coord);
 }
 
@@ -177,7 +189,7 @@ int ld_advance_plain(ld_ctx* ctx, double h, int32_t nsteps) {
 }
 
 int ld_step(ld_ctx* ctx, double h, void* uk, void* n, double* coord, int64_t np, int32_t nsteps) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(n);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(uk); LD_FIELD(n);
   if (np < 0 || nsteps < 0) { set_error("np and nsteps must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) LD_NONNULL(coord);
   int rc = ctx->eng->state_set(uk, n, coord, np);
@@ -216,18 +228,18 @@ int ld_profile_read(ld_ctx* ctx, double* total_ms, int64_t* launches) {
 }
 
 int ld_spectrum(ld_ctx* ctx, const void* f, int32_t nfields, double weight, double* out) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(f); LD_NONNULL(out);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(f); LD_NONNULL(out);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->spectrum(f, nfields, weight, out);
 }
 
 int ld_flux(ld_ctx* ctx, const void* uk, const void* ku, double* out) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(uk); LD_NONNULL(ku); LD_NONNULL(out);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(uk); LD_FIELD(ku); LD_NONNULL(out);
   return ctx->eng->flux(uk, ku, out);
 }
 
 int ld_field_stats(ld_ctx* ctx, const void* x, int32_t nfields, double* out) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(x); LD_NONNULL(out);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(x); LD_NONNULL(out);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->stats(x, nfields, out);
 }
@@ -259,13 +271,13 @@ int ld_coalesce(ld_ctx* ctx, double* rows, int64_t np, const int64_t* pairs, int
 int ld_interp_cosine(ld_ctx* ctx, const double* coord, int64_t np, const void* fields, int32_t ncomp, double* out) {
   LD_CHECK_CTX(ctx); LD_ONE(ctx);
   if (np < 0 || ncomp < 1) { set_error("np must be >= 0 and ncomp >= 1"); return LD_ERR_ARG; }
-  if (np > 0) { LD_NONNULL(coord); LD_NONNULL(fields); LD_NONNULL(out); }
+  if (np > 0) { LD_NONNULL(coord); LD_FIELD(fields); LD_NONNULL(out); }
   if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
   return ctx->eng->interp(coord, np, fields, ncomp, out);
 }
 
 int ld_exterp_cosine_scalar(ld_ctx* ctx, const double* coord, int64_t np, const double* w, void* c) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(c);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(c);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(w); }
   if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
@@ -274,7 +286,7 @@ int ld_exterp_cosine_scalar(ld_ctx* ctx, const double* coord, int64_t np, const 
 
 int ld_prhs(ld_ctx* ctx, const double* coord, int64_t np, const void* u, const void* us, const void* n, void* fc,
             double* rhs, double* interpmat, double* exterpmat) {
-  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(u); LD_NONNULL(us); LD_NONNULL(n); LD_NONNULL(fc);
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(u); LD_FIELD(us); LD_FIELD(n); LD_FIELD(fc);
   if (np < 0) { set_error("np must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) { LD_NONNULL(coord); LD_NONNULL(rhs); LD_NONNULL(interpmat); LD_NONNULL(exterpmat); }
   if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }

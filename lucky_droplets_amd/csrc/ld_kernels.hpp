@@ -867,13 +867,31 @@ k_sum2(const T* __restrict__ x0, size_t n, double* __restrict__ partials0, size_
   double* __restrict__ partials = rsel1(partials0, rs);
   constexpr int NT = 256;
   __shared__ double sh[NT];
-  const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const size_t b = blockIdx.x * chunk, e = (b + chunk < n) ? b + chunk : n;
+  // block b sums quads [qb, qe) of 4 consecutive entries (16-byte FP32
+  // loads), four quads per thread in flight; the last block adds the n % 4 tail
+  const size_t nq = n / 4, chunk = (nq + gridDim.x - 1) / gridDim.x;
+  const size_t qb = blockIdx.x * chunk, qe = (qb + chunk < nq) ? qb + chunk : nq;
+  const V4<T>* xq = reinterpret_cast<const V4<T>*>(x);
   double s0 = 0.0, s1 = 0.0;
-  for (size_t i = b + threadIdx.x; i < e; i += NT) {
-    const double v = (double)x[i];
-    s0 += v; s1 += v > 0.0 ? v : 0.0;
+  constexpr int U = 4;
+  for (size_t i0 = qb + threadIdx.x; i0 < qe; i0 += U * NT) {
+    V4<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * NT < qe) v[u] = xq[i0 + u * NT];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * NT < qe) {
+        const double a[4] = {(double)v[u].x, (double)v[u].y, (double)v[u].z, (double)v[u].w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { s0 += a[k]; s1 += a[k] > 0.0 ? a[k] : 0.0; }
+      }
   }
+  if (blockIdx.x == gridDim.x - 1)
+    for (size_t i = 4 * nq + threadIdx.x; i < n; i += NT) {
+      const double v = (double)x[i];
+      s0 += v; s1 += v > 0.0 ? v : 0.0;
+    }
   const double a = blockSum<NT>(s0, sh);
   if (threadIdx.x == 0) partials[2 * blockIdx.x] = a;
   __syncthreads();
@@ -902,11 +920,35 @@ __global__ void k_clip_apply(T* x0, T* out0, size_t n, const double* __restrict_
   T* x = rsel1(x0, rs);
   T* out = rsel1(out0, rs);
   const double* __restrict__ clip = rsel1(clip0, rs);
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const double m0 = clip[0], m1 = clip[1];
+  // quads of 4 entries (16-byte FP32 accesses), four per thread in flight
+  const size_t nq = n / 4, stride = (size_t)gridDim.x * blockDim.x;
+  V4<T>* xq = reinterpret_cast<V4<T>*>(x);
+  V4<T>* oq = reinterpret_cast<V4<T>*>(out);
+  constexpr int U = 4;
+  for (size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i0 < nq; i0 += U * stride) {
+    V4<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * stride < nq) v[u] = xq[i0 + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * stride < nq) {
+        V4<T> c;
+        c.x = v[u].x > (T)0 ? v[u].x : (T)0; c.y = v[u].y > (T)0 ? v[u].y : (T)0;
+        c.z = v[u].z > (T)0 ? v[u].z : (T)0; c.w = v[u].w > (T)0 ? v[u].w : (T)0;
+        if (clip_x) xq[i0 + u * stride] = c;
+        V4<T> r;
+        r.x = (T)(((double)c.x * m0) / m1); r.y = (T)(((double)c.y * m0) / m1);
+        r.z = (T)(((double)c.z * m0) / m1); r.w = (T)(((double)c.w * m0) / m1);
+        oq[i0 + u * stride] = r;
+      }
+  }
+  for (size_t i = 4 * nq + blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += stride) {
     const T v = x[i];
     const T c = v > (T)0 ? v : (T)0;
     if (clip_x) x[i] = c;
-    out[i] = (T)(((double)c * clip[0]) / clip[1]);
+    out[i] = (T)(((double)c * m0) / m1);
   }
 }
 

@@ -69,6 +69,32 @@ def test_forcing(gpu, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_clip_zero(gpu, prec):
+    """ld_clip_zero: x clipped in place, out = x * oldmean / newmean; misaligned
+    field pointers are refused."""
+    import ctypes
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(32, prec)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((32, 32, 32)) + 0.5
+    fl = make_flow_oracle(cfg)
+    xr = x.copy()
+    ref = fl.clip_zero(xr)
+    d = ForcedDNS(cfg, max_particles=0)
+    xg = _t(x, gpu, d.rdtype)
+    out = d.clip_zero(xg)
+    tol = 1e-14 if prec == "f64" else 1e-6
+    assert relerr(out.cpu().numpy(), ref) < tol
+    assert relerr(xg.cpu().numpy(), xr) < tol
+    raw = torch.zeros(32 ** 3 + 8, dtype=d.rdtype, device=gpu)
+    rc = d.lib.ld_clip_zero(d.ctx, ctypes.c_void_p(raw.data_ptr() + raw.element_size()),
+                            ctypes.c_void_p(raw.data_ptr()))
+    assert rc == 1
+    d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("nparts", [0, 40])
 def test_full_rhs(gpu, prec, nparts):
     import torch
