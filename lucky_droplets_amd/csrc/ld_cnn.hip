@@ -59,9 +59,37 @@ struct ld_cnn {
 namespace {
 
 // Convolution launch for the channel pairs the network uses (16 or 32 each).
+// gfx950 double-rate form: NT node tiles of 16 per wavefront (a z-line
+// segment of 16 NT nodes), persistent blocks of 4 wavefronts.
+template <int CI, int CO, bool FLIP, bool RELU, bool MASK>
+int conv2(ld_cnn* c, const bf16* X, const bf16* W, const float* b, const bf16* mask, bf16* Yb, float* Yf) {
+  const int N = c->N;
+  const int NT = N >= 128 ? 8 : N / 16;
+  const int nseg = (int)(c->nodes / (16 * NT));
+  int G = (nseg + 3) / 4;
+  if (G > 2048) G = 2048;
+  const dim3 g((unsigned)G), t(256);
+#define LD_NT_CASE(K)                                                                                          \
+  if (NT == K) {                                                                                               \
+    hipLaunchKernelGGL((k_conv2<CI, CO, FLIP, RELU, MASK, K>), g, t, 0, c->st, X, W, b, mask, Yb, Yf, N, nseg); \
+    LAUNCHED();                                                                                                \
+    return LD_OK;                                                                                              \
+  }
+  LD_NT_CASE(1) LD_NT_CASE(2) LD_NT_CASE(4) LD_NT_CASE(8)
+#undef LD_NT_CASE
+  set_error("conv2: unsupported N %d", N);
+  return LD_ERR_ARG;
+}
+
 template <bool FLIP, bool RELU, bool MASK>
 int conv(ld_cnn* c, int CI, int CO, const bf16* X, const bf16* W, const float* b, const bf16* mask, bf16* Yb,
          float* Yf) {
+  if (c->N >= 16) {
+    if (CI == 16 && CO == 16) return conv2<16, 16, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf);
+    if (CI == 16 && CO == 32) return conv2<16, 32, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf);
+    if (CI == 32 && CO == 16) return conv2<32, 16, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf);
+    if (CI == 32 && CO == 32) return conv2<32, 32, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf);
+  }
   const dim3 g((unsigned)(c->nodes / 64)), t(256);
 #define LD_CONV_CASE(A, B)                                                                                    \
   if (CI == A && CO == B) {                                                                                   \

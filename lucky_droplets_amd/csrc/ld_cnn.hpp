@@ -86,6 +86,108 @@ k_conv(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __re
   }
 }
 
+// The same convolution with the gfx950 double-rate instruction
+// __builtin_amdgcn_mfma_f32_16x16x32_bf16, oriented M = output channels,
+// N = nodes, K = (tap, input channel) pairs in steps of 32:
+//   A (16x32): lane l holds W[tap][co0 + l % 16][ci0 + i]       (8 bf16, 16 B)
+//   B (32x16): lane l holds X[nbr(node0 + l % 16, tap)][ci0 + i] (8 bf16, 16 B)
+//   D (16x16): lane l holds Y[node0 + l % 16][co0 + 4 (l / 16) + i], i < 4
+// with (tap, ci0) = divmod(32 ks + 8 (l / 16), CI) for K-step ks (CI = 16: a
+// step covers two taps; the 28th tap of the last step has zero weights).
+// The layer's weights are staged in LDS once per block; a wavefront computes
+// NT tiles of 16 consecutive nodes along z (one z-line segment) for all CO,
+// so each weight fragment feeds NT instructions and each input fragment CO/16.
+// Blocks loop over segments (persistent grid), XCD-aware: each XCD takes a
+// contiguous run of blocks, so neighbouring lines share its L2.
+// Needs N >= 16 * NT and N a multiple of 16 * NT.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int CI, int CO, bool FLIP, bool RELU, bool MASK, int NT>
+__global__ void __launch_bounds__(256)
+k_conv2(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __restrict__ bias,
+        const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N, int nseg) {
+  static_assert(CI % 16 == 0 && CO % 16 == 0, "channels: multiples of 16");
+  constexpr int MT = CO / 16;
+  constexpr int KS = (27 * CI + 31) / 32;
+  constexpr int WN = 27 * CO * CI;
+  __shared__ __attribute__((aligned(16))) bf16 sW[WN];
+  for (int e = threadIdx.x; e < WN / 8; e += 256)
+    reinterpret_cast<uint4*>(sW)[e] = reinterpret_cast<const uint4*>(W)[e];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n16 = lane & 15, q = lane >> 4;
+  const int G = gridDim.x;
+  const int lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int spl = N / (16 * NT);                  // segments per z-line
+  const int msk = N - 1;
+  for (int seg = lb * 4 + wv; seg < nseg; seg += G * 4) {
+    const int line = seg / spl, z0 = (seg % spl) * 16 * NT;
+    const int x = line / N, y = line & msk;
+    f32x4 acc[NT][MT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) acc[t][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = 32 * ks + 8 * q;
+      const int tap0 = k / CI, ci0 = k % CI;
+      const bool valid = tap0 < 27;
+      const int tap = valid ? tap0 : 26;
+      int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+      if (FLIP) { dx = -dx; dy = -dy; dz = -dz; }
+      bf16x8 wf[MT], xf[NT];
+      // opaque to the compiler, so the weight fragments are re-read from LDS
+      // per segment instead of being hoisted into registers for all K-steps
+      int wofs = (tap * CO + n16) * CI + ci0;
+      asm volatile("" : "+v"(wofs));
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const uint4 v = *reinterpret_cast<const uint4*>(sW + wofs + m * 16 * CI);
+        wf[m] = valid ? __builtin_bit_cast(bf16x8, v) : bf16x8{};
+      }
+      const size_t row = ((size_t)((x + dx) & msk) * N + ((y + dy) & msk)) * N;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int zz = (z0 + 16 * t + n16 + dz) & msk;
+        const uint4 v = *reinterpret_cast<const uint4*>(X + (row + zz) * CI + ci0);
+        xf[t] = valid ? __builtin_bit_cast(bf16x8, v) : bf16x8{};
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[m], xf[t], acc[t][m], 0, 0, 0);
+    }
+    const size_t nbase = (size_t)line * N + z0 + n16;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const int co = m * 16 + 4 * q;
+        const size_t o = (nbase + 16 * t) * CO + co;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          v[i] = acc[t][m][i] + (bias ? bias[co + i] : 0.f);
+          if (RELU) v[i] = v[i] > 0.f ? v[i] : 0.f;
+        }
+        if (MASK) {
+          const uint2 mk = *reinterpret_cast<const uint2*>(mask + o);
+          const bf16 mb[4] = {(bf16)(mk.x & 0xffff), (bf16)(mk.x >> 16), (bf16)(mk.y & 0xffff), (bf16)(mk.y >> 16)};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = b2f(mb[i]) > 0.f ? v[i] : 0.f;
+        }
+        if (Yb) {
+          uint2 pk;
+          pk.x = (uint32_t)f2b(v[0]) | ((uint32_t)f2b(v[1]) << 16);
+          pk.y = (uint32_t)f2b(v[2]) | ((uint32_t)f2b(v[3]) << 16);
+          *reinterpret_cast<uint2*>(Yb + o) = pk;
+        }
+        if (Yf) *reinterpret_cast<float4*>(Yf + o) = make_float4(v[0], v[1], v[2], v[3]);
+      }
+  }
+}
+
 // Weight and bias gradients of one layer, per block over `chunk` nodes:
 //   part[blk][(tap CO + co) CI + ci] = sum_node dZ[node][co] X[nbr(node, tap)][ci]
 //   part[blk][27 CO CI + co]         = sum_node dZ[node][co]
