@@ -7,6 +7,9 @@
 #include <vector>
 
 using namespace ld::cnn;
+
+// blocks of the loss kernel (one FP64 partial sum each)
+#define LD_CNN_MSE_BLOCKS 2048
 using ld::set_error;
 
 struct ld_cnn {
@@ -26,6 +29,8 @@ struct ld_cnn {
   float* part = nullptr;
   double* mpart = nullptr;
   int nblk_w = 0, chunk = 0;
+  int nblk_w2 = 0, chunk2 = 0;              // k_wgrad2 blocks and nodes per block (N >= 16)
+  bf16 *xt = nullptr, *dzt = nullptr;       // channel-major copies for the weight gradient
   int adam_t = 0;
   bool packed = false;
   ~ld_cnn() {
@@ -103,7 +108,32 @@ int conv(ld_cnn* c, int CI, int CO, const bf16* X, const bf16* W, const float* b
   return LD_ERR_ARG;
 }
 
-int wgrad(ld_cnn* c, int CI, int CO, const bf16* X, const bf16* dZ) {
+template <int C> int cmajor(ld_cnn* c, const bf16* A, bf16* AT) {
+  hipLaunchKernelGGL((k_cmajor<C>), dim3((unsigned)(c->nodes / 256)), dim3(256), 0, c->st, A, AT, c->nodes);
+  LAUNCHED();
+  return LD_OK;
+}
+
+// Weight and bias gradients of one layer into c->part (per-block partials);
+// returns the number of partial blocks in *nblk.
+int wgrad(ld_cnn* c, int CI, int CO, const bf16* X, const bf16* dZ, int* nblk) {
+  if (c->N >= 16) {
+    CK(CI == 16 ? cmajor<16>(c, X, c->xt) : cmajor<32>(c, X, c->xt));
+    CK(CO == 16 ? cmajor<16>(c, dZ, c->dzt) : cmajor<32>(c, dZ, c->dzt));
+    const dim3 g((unsigned)c->nblk_w2), t(576);
+#define LD_WG2_CASE(A, B)                                                                                  \
+    if (CI == A && CO == B) {                                                                              \
+      hipLaunchKernelGGL((k_wgrad2<A, B>), g, t, 0, c->st, c->xt, c->dzt, c->part, c->N, c->chunk2);     \
+      LAUNCHED();                                                                                          \
+      *nblk = c->nblk_w2;                                                                                  \
+      return LD_OK;                                                                                        \
+    }
+    LD_WG2_CASE(16, 16) LD_WG2_CASE(16, 32) LD_WG2_CASE(32, 16) LD_WG2_CASE(32, 32)
+#undef LD_WG2_CASE
+    set_error("unsupported channel pair %d -> %d", CI, CO);
+    return LD_ERR_ARG;
+  }
+  *nblk = c->nblk_w;
   const dim3 g((unsigned)c->nblk_w), t(256);
 #define LD_WG_CASE(A, B)                                                                             \
   if (CI == A && CO == B) {                                                                          \
@@ -131,7 +161,9 @@ int pack_weights(ld_cnn* c) {
 // input for the backward pass; FP32 output in c->yp.
 int forward(ld_cnn* c, const float* x) {
   if (!c->packed) CK(pack_weights(c));
-  hipLaunchKernelGGL(k_pack_input, dim3(1024), dim3(256), 0, c->st, x, c->cin, c->ch[0], c->nodes, c->act[0]);
+  const size_t gp = (c->nodes + 255) / 256;
+  hipLaunchKernelGGL(k_pack_input, dim3((unsigned)(gp < (1u << 20) ? gp : (1u << 20))), dim3(256), 0, c->st, x, c->cin,
+                     c->ch[0], c->nodes, c->act[0]);
   LAUNCHED();
   for (int l = 0; l < c->L; ++l) {
     const float* b = c->params + c->off[l] + 27LL * c->ch[l + 1] * c->ch[l];
@@ -146,16 +178,17 @@ int forward(ld_cnn* c, const float* x) {
 // Mean-squared-error loss against target[node][cout] and the parameter
 // gradients (in c->grads) by backpropagation through the stored activations.
 int backward(ld_cnn* c, const float* target, double* loss) {
-  const int nb = 256;
+  const int nb = LD_CNN_MSE_BLOCKS;
   hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, c->st, (const float*)c->yp, target, c->cout, c->ch[c->L], c->nodes,
                      c->dz[0], c->mpart);
   LAUNCHED();
   int cur = 0;
   for (int l = c->L - 1; l >= 0; --l) {
     const int CI = c->ch[l], CO = c->ch[l + 1];
-    CK(wgrad(c, CI, CO, c->act[l], c->dz[cur]));
+    int nblk = 0;
+    CK(wgrad(c, CI, CO, c->act[l], c->dz[cur], &nblk));
     const int n = 27 * CO * CI + CO;
-    hipLaunchKernelGGL(k_reduce_parts, dim3((n + 255) / 256), dim3(256), 0, c->st, (const float*)c->part, c->nblk_w, n,
+    hipLaunchKernelGGL(k_reduce_parts, dim3((n + 255) / 256), dim3(256), 0, c->st, (const float*)c->part, nblk, n,
                        c->grads + c->off[l]);
     LAUNCHED();
     if (l > 0) {
@@ -212,6 +245,10 @@ int ld_cnn_create(const ld_cnn_params* p, ld_cnn** out) {
   c->chunk = 64;
   while (c->nodes / c->chunk > 256) c->chunk *= 2;
   c->nblk_w = (int)(c->nodes / c->chunk);
+  // k_wgrad2: at most 512 blocks, each over a multiple of 32 nodes
+  c->chunk2 = 32;
+  while (c->nodes / c->chunk2 > 512) c->chunk2 *= 2;
+  c->nblk_w2 = (int)(c->nodes / c->chunk2);
   int64_t maxlayer = 0;
   for (int l = 0; l < c->L; ++l) {
     const int64_t n = 27LL * c->ch[l] * c->ch[l + 1] + c->ch[l + 1];
@@ -229,8 +266,10 @@ int ld_cnn_create(const ld_cnn_params* p, ld_cnn** out) {
   if (rc == LD_OK) rc = c->alloc(c->dz[0], c->nodes * 32);
   if (rc == LD_OK) rc = c->alloc(c->dz[1], c->nodes * 32);
   if (rc == LD_OK) rc = c->alloc(c->yp, c->nodes * c->ch[c->L]);
-  if (rc == LD_OK) rc = c->alloc(c->part, (size_t)c->nblk_w * maxlayer);
-  if (rc == LD_OK) rc = c->alloc(c->mpart, 256);
+  if (rc == LD_OK) rc = c->alloc(c->part, (size_t)(c->nblk_w > c->nblk_w2 ? c->nblk_w : c->nblk_w2) * maxlayer);
+  if (rc == LD_OK && c->N >= 16) rc = c->alloc(c->xt, c->nodes * 32);
+  if (rc == LD_OK && c->N >= 16) rc = c->alloc(c->dzt, c->nodes * 32);
+  if (rc == LD_OK) rc = c->alloc(c->mpart, LD_CNN_MSE_BLOCKS);
   if (rc == LD_OK && hipMemset(c->params, 0, np * sizeof(float)) != hipSuccess) rc = LD_ERR_HIP;
   if (rc == LD_OK && hipMemset(c->m, 0, np * sizeof(float)) != hipSuccess) rc = LD_ERR_HIP;
   if (rc == LD_OK && hipMemset(c->v, 0, np * sizeof(float)) != hipSuccess) rc = LD_ERR_HIP;

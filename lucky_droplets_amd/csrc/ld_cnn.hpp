@@ -262,22 +262,161 @@ k_wgrad(const bf16* __restrict__ X, const bf16* __restrict__ dZ, float* __restri
   if (threadIdx.x < CO) out[27 * CO * CI + threadIdx.x] = db;
 }
 
+// Channels-last A[node][C] -> channel-major AT[C][node] (bf16), 256 nodes per
+// block through LDS: 16-byte row reads, 16-byte (8-node) column writes.
+template <int C>
+__global__ void __launch_bounds__(256)
+k_cmajor(const bf16* __restrict__ A, bf16* __restrict__ AT, size_t nodes) {
+  constexpr int NB = 256, RS = C + 2;           // LDS row stride (elements), odd dwords: no bank conflicts
+  __shared__ bf16 t[NB * RS];
+  const size_t n0 = (size_t)blockIdx.x * NB;
+  for (int e = threadIdx.x; e < NB * C / 8; e += 256) {
+    const int r = e / (C / 8), c8 = (e % (C / 8)) * 8;
+    const uint4 v = *reinterpret_cast<const uint4*>(A + (n0 + r) * C + c8);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      t[r * RS + c8 + 2 * i] = (bf16)(w[i] & 0xffff);
+      t[r * RS + c8 + 2 * i + 1] = (bf16)(w[i] >> 16);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < C * NB / 8; e += 256) {
+    const int c = e / (NB / 8), j = (e % (NB / 8)) * 8;
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      w[i] = (uint32_t)t[(j + 2 * i) * RS + c] | ((uint32_t)t[(j + 2 * i + 1) * RS + c] << 16);
+    *reinterpret_cast<uint4*>(AT + (size_t)c * nodes + n0 + j) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// Weight and bias gradients from channel-major operands on the 16x16x32 bf16
+// MFMA (M = output channels, N = input channels, K = nodes):
+//   A (16x32): lane l holds dZ[node k0 + 8 (l / 16) + i][co0 + l % 16]   = dZT[co][8 consecutive nodes]
+//   B (32x16): lane l holds X[nbr(node k0 + 8 (l / 16) + i, tap)][ci0 + l % 16]
+// Eight consecutive nodes of a lane lie on one z-line, so their neighbours for
+// tap (dx, dy, dz) are eight consecutive z on the neighbour line, shifted by
+// dz: the lane loads the aligned 16 bytes at z plus one dword on each side
+// (periodic in z) and forms the dz = -1 and +1 operands with v_alignbyte.
+// Wavefront w of the 9 in a block owns the neighbour line (dx, dy) =
+// (w / 3 - 1, w % 3 - 1), i.e. taps 3 w .. 3 w + 2.  Block b sums nodes
+// [b chunk, (b + 1) chunk) in K-steps of 32 and writes its partial
+//   part[b][(tap CO + co) CI + ci],  part[b][27 CO CI + co] (bias, wavefront 0).
+template <int CI, int CO>
+__global__ void __launch_bounds__(576)
+k_wgrad2(const bf16* __restrict__ XT, const bf16* __restrict__ dZT, float* __restrict__ part, int N, int chunk) {
+  constexpr int MT = CO / 16, NT = CI / 16;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n16 = lane & 15, q = lane >> 4;
+  const int dx = w / 3 - 1, dy = w % 3 - 1, msk = N - 1;
+  const size_t nodes = (size_t)N * N * N;
+  f32x4 acc[3][MT][NT];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[d][m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float db[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) db[m] = 0.f;
+  const size_t kb = (size_t)blockIdx.x * chunk;
+#pragma unroll 2
+  for (int ks = 0; ks < chunk / 32; ++ks) {
+    const size_t nq = kb + 32 * ks + 8 * q;
+    const int zq = (int)(nq & msk);
+    const size_t line = nq / N;
+    const int x = (int)(line / N), y = (int)(line & msk);
+    const size_t base = ((size_t)((x + dx) & msk) * N + ((y + dy) & msk)) * N;
+    bf16x8 a[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const uint4 v = *reinterpret_cast<const uint4*>(dZT + (size_t)(m * 16 + n16) * nodes + nq);
+      a[m] = __builtin_bit_cast(bf16x8, v);
+      if (w == 0) {
+        const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) db[m] += b2f((bf16)(u[i] & 0xffff)) + b2f((bf16)(u[i] >> 16));
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const bf16* row = XT + (size_t)(t * 16 + n16) * nodes + base;
+      const uint4 c = *reinterpret_cast<const uint4*>(row + zq);
+      const uint32_t pv = *reinterpret_cast<const uint32_t*>(row + ((zq - 2) & msk));
+      const uint32_t nv = *reinterpret_cast<const uint32_t*>(row + ((zq + 8) & msk));
+      const uint32_t s1 = __builtin_amdgcn_alignbyte(c.y, c.x, 2), s2 = __builtin_amdgcn_alignbyte(c.z, c.y, 2),
+                     s3 = __builtin_amdgcn_alignbyte(c.w, c.z, 2);
+      const bf16x8 fm = __builtin_bit_cast(bf16x8, make_uint4(__builtin_amdgcn_alignbyte(c.x, pv, 2), s1, s2, s3));
+      const bf16x8 f0 = __builtin_bit_cast(bf16x8, c);
+      const bf16x8 fp = __builtin_bit_cast(bf16x8, make_uint4(s1, s2, s3, __builtin_amdgcn_alignbyte(nv, c.w, 2)));
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        acc[0][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], fm, acc[0][m][t], 0, 0, 0);
+        acc[1][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], f0, acc[1][m][t], 0, 0, 0);
+        acc[2][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], fp, acc[2][m][t], 0, 0, 0);
+      }
+    }
+  }
+  float* out = part + (size_t)blockIdx.x * (27 * CO * CI + CO);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int tap = 3 * w + d;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          out[((size_t)tap * CO + m * 16 + 4 * q + i) * CI + t * 16 + n16] = acc[d][m][t][i];
+  }
+  if (w == 0) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      float v = db[m];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (q == 0) out[27 * CO * CI + m * 16 + n16] = v;
+    }
+  }
+}
+
 // out[i] = sum over blocks of part[b][i] (fixed order), i < n.
 static __global__ void __launch_bounds__(256)
 k_reduce_parts(const float* __restrict__ part, int nblk, int n, float* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // loads in batches of 16 ahead of their (ordered) additions
   float s = 0.f;
-  for (int b = 0; b < nblk; ++b) s += part[(size_t)b * n + i];
+  int b = 0;
+  for (; b + 16 <= nblk; b += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = part[(size_t)(b + u) * n + i];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += v[u];
+  }
+  for (; b < nblk; ++b) s += part[(size_t)b * n + i];
   out[i] = s;
 }
 
 // x[node][cin] FP32 -> A[node][CP] bf16 (channels cin..CP-1 zero).
+// One node per thread and step, 16-byte stores (CP is 16 or 32).
 static __global__ void k_pack_input(const float* __restrict__ x, int cin, int CP, size_t nodes, bf16* __restrict__ A) {
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nodes * CP; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t node = e / CP;
-    const int c = (int)(e % CP);
-    A[e] = c < cin ? f2b(x[node * cin + c]) : (bf16)0;
+  for (size_t node = blockIdx.x * (size_t)blockDim.x + threadIdx.x; node < nodes;
+       node += (size_t)gridDim.x * blockDim.x) {
+    for (int c8 = 0; c8 < CP; c8 += 8) {
+      uint32_t w[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c = c8 + 2 * i;
+        const uint32_t lo = c < cin ? f2b(x[node * cin + c]) : 0u;
+        const uint32_t hi = c + 1 < cin ? f2b(x[node * cin + c + 1]) : 0u;
+        w[i] = lo | (hi << 16);
+      }
+      *reinterpret_cast<uint4*>(A + node * CP + c8) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
   }
 }
 
@@ -301,16 +440,25 @@ k_mse(const float* __restrict__ y, const float* __restrict__ t, int cout, int CP
   __shared__ double sh[256];
   double s = 0.0;
   const double scale = 2.0 / ((double)nodes * cout);
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nodes * CP; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t node = e / CP;
-    const int c = (int)(e % CP);
-    float g = 0.f;
-    if (c < cout) {
-      const double d = (double)y[e] - (double)t[node * cout + c];
-      s += d * d;
-      g = (float)(scale * d);
+  // one node per thread and step, 16-byte stores of dY (CP is 16 or 32)
+  for (size_t node = blockIdx.x * (size_t)blockDim.x + threadIdx.x; node < nodes;
+       node += (size_t)gridDim.x * blockDim.x) {
+    for (int c8 = 0; c8 < CP; c8 += 8) {
+      uint32_t w[4];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int c = c8 + i;
+        float g = 0.f;
+        if (c < cout) {
+          const double d = (double)y[node * CP + c] - (double)t[node * cout + c];
+          s += d * d;
+          g = (float)(scale * d);
+        }
+        const uint32_t b = f2b(g);
+        if (i & 1) w[i / 2] |= b << 16; else w[i / 2] = b;
+      }
+      *reinterpret_cast<uint4*>(dY + node * CP + c8) = make_uint4(w[0], w[1], w[2], w[3]);
     }
-    dY[e] = f2b(g);
   }
   sh[threadIdx.x] = s;
   __syncthreads();
