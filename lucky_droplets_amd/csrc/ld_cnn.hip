@@ -30,7 +30,7 @@ struct ld_cnn {
   double* mpart = nullptr;
   int nblk_w = 0, chunk = 0;
   int nblk_w2 = 0, chunk2 = 0;              // k_wgrad2 blocks and nodes per block (N >= 16)
-  bf16 *xt = nullptr, *dzt = nullptr;       // channel-major copies for the weight gradient
+  bf16 *xt = nullptr, *dzt = nullptr;       // node-blocked channel-major copies for the weight gradient
   int adam_t = 0;
   bool packed = false;
   ~ld_cnn() {
@@ -109,7 +109,8 @@ int conv(ld_cnn* c, int CI, int CO, const bf16* X, const bf16* W, const float* b
 }
 
 template <int C> int cmajor(ld_cnn* c, const bf16* A, bf16* AT) {
-  hipLaunchKernelGGL((k_cmajor<C>), dim3((unsigned)(c->nodes / 256)), dim3(256), 0, c->st, A, AT, c->nodes);
+  const size_t n = c->nodes / 8 * (C / 8);
+  hipLaunchKernelGGL((k_cblock<C>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, A, AT, c->nodes);
   LAUNCHED();
   return LD_OK;
 }

@@ -262,39 +262,44 @@ k_wgrad(const bf16* __restrict__ X, const bf16* __restrict__ dZ, float* __restri
   if (threadIdx.x < CO) out[27 * CO * CI + threadIdx.x] = db;
 }
 
-// Channels-last A[node][C] -> channel-major AT[C][node] (bf16), 256 nodes per
-// block through LDS: 16-byte row reads, 16-byte (8-node) column writes.
+// Channels-last A[node][C] -> node-blocked channel-major AB (bf16): entry
+// (node, c) at ((node / 8) C + c) 8 + node % 8, i.e. for each group of 8
+// nodes, C runs of 8 consecutive nodes (16 bytes each).  One thread per group
+// and 8 channels: eight 16-byte loads, an 8 x 8 transpose in registers,
+// eight 16-byte stores.
 template <int C>
 __global__ void __launch_bounds__(256)
-k_cmajor(const bf16* __restrict__ A, bf16* __restrict__ AT, size_t nodes) {
-  constexpr int NB = 256, RS = C + 2;           // LDS row stride (elements), odd dwords: no bank conflicts
-  __shared__ bf16 t[NB * RS];
-  const size_t n0 = (size_t)blockIdx.x * NB;
-  for (int e = threadIdx.x; e < NB * C / 8; e += 256) {
-    const int r = e / (C / 8), c8 = (e % (C / 8)) * 8;
-    const uint4 v = *reinterpret_cast<const uint4*>(A + (n0 + r) * C + c8);
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+k_cblock(const bf16* __restrict__ A, bf16* __restrict__ AB, size_t nodes) {
+  constexpr int CG = C / 8;
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= nodes / 8 * CG) return;
+  const size_t g = e / CG;
+  const int c8 = (int)(e % CG) * 8;
+  uint32_t r[8][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      t[r * RS + c8 + 2 * i] = (bf16)(w[i] & 0xffff);
-      t[r * RS + c8 + 2 * i + 1] = (bf16)(w[i] >> 16);
-    }
+  for (int i = 0; i < 8; ++i) {
+    const uint4 v = *reinterpret_cast<const uint4*>(A + (g * 8 + i) * C + c8);
+    r[i][0] = v.x; r[i][1] = v.y; r[i][2] = v.z; r[i][3] = v.w;
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < C * NB / 8; e += 256) {
-    const int c = e / (NB / 8), j = (e % (NB / 8)) * 8;
-    uint32_t w[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      w[i] = (uint32_t)t[(j + 2 * i) * RS + c] | ((uint32_t)t[(j + 2 * i + 1) * RS + c] << 16);
-    *reinterpret_cast<uint4*>(AT + (size_t)c * nodes + n0 + j) = make_uint4(w[0], w[1], w[2], w[3]);
+  for (int j = 0; j < 8; ++j) {
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t a = (r[2 * k][j / 2] >> (16 * (j & 1))) & 0xffffu;
+      const uint32_t b = (r[2 * k + 1][j / 2] >> (16 * (j & 1))) & 0xffffu;
+      o[k] = a | (b << 16);
+    }
+    *reinterpret_cast<uint4*>(AB + (g * C + c8 + j) * 8) = make_uint4(o[0], o[1], o[2], o[3]);
   }
 }
 
-// Weight and bias gradients from channel-major operands on the 16x16x32 bf16
-// MFMA (M = output channels, N = input channels, K = nodes):
-//   A (16x32): lane l holds dZ[node k0 + 8 (l / 16) + i][co0 + l % 16]   = dZT[co][8 consecutive nodes]
+// Weight and bias gradients from node-blocked channel-major operands
+// (k_cblock layout) on the 16x16x32 bf16 MFMA (M = output channels,
+// N = input channels, K = nodes):
+//   A (16x32): lane l holds dZ[node k0 + 8 (l / 16) + i][co0 + l % 16]  (one 16-byte run)
 //   B (32x16): lane l holds X[nbr(node k0 + 8 (l / 16) + i, tap)][ci0 + l % 16]
+// so the 16 lanes of a row read 256 consecutive bytes.
 // Eight consecutive nodes of a lane lie on one z-line, so their neighbours for
 // tap (dx, dy, dz) are eight consecutive z on the neighbour line, shifted by
 // dz: the lane loads the aligned 16 bytes at z plus one dword on each side
@@ -305,12 +310,11 @@ k_cmajor(const bf16* __restrict__ A, bf16* __restrict__ AT, size_t nodes) {
 //   part[b][(tap CO + co) CI + ci],  part[b][27 CO CI + co] (bias, wavefront 0).
 template <int CI, int CO>
 __global__ void __launch_bounds__(576)
-k_wgrad2(const bf16* __restrict__ XT, const bf16* __restrict__ dZT, float* __restrict__ part, int N, int chunk) {
+k_wgrad2(const bf16* __restrict__ XB, const bf16* __restrict__ dZB, float* __restrict__ part, int N, int chunk) {
   constexpr int MT = CO / 16, NT = CI / 16;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n16 = lane & 15, q = lane >> 4;
   const int dx = w / 3 - 1, dy = w % 3 - 1, msk = N - 1;
-  const size_t nodes = (size_t)N * N * N;
   f32x4 acc[3][MT][NT];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
@@ -332,7 +336,7 @@ k_wgrad2(const bf16* __restrict__ XT, const bf16* __restrict__ dZT, float* __res
     bf16x8 a[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
-      const uint4 v = *reinterpret_cast<const uint4*>(dZT + (size_t)(m * 16 + n16) * nodes + nq);
+      const uint4 v = *reinterpret_cast<const uint4*>(dZB + ((nq >> 3) * CO + m * 16 + n16) * 8);
       a[m] = __builtin_bit_cast(bf16x8, v);
       if (w == 0) {
         const uint32_t u[4] = {v.x, v.y, v.z, v.w};
@@ -342,10 +346,11 @@ k_wgrad2(const bf16* __restrict__ XT, const bf16* __restrict__ dZT, float* __res
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const bf16* row = XT + (size_t)(t * 16 + n16) * nodes + base;
-      const uint4 c = *reinterpret_cast<const uint4*>(row + zq);
-      const uint32_t pv = *reinterpret_cast<const uint32_t*>(row + ((zq - 2) & msk));
-      const uint32_t nv = *reinterpret_cast<const uint32_t*>(row + ((zq + 8) & msk));
+      const int ci = t * 16 + n16;
+      const size_t np = base + ((zq - 2) & msk), nn = base + ((zq + 8) & msk);
+      const uint4 c = *reinterpret_cast<const uint4*>(XB + (((base + zq) >> 3) * CI + ci) * 8);
+      const uint32_t pv = *reinterpret_cast<const uint32_t*>(XB + ((np >> 3) * CI + ci) * 8 + (np & 7));
+      const uint32_t nv = *reinterpret_cast<const uint32_t*>(XB + ((nn >> 3) * CI + ci) * 8 + (nn & 7));
       const uint32_t s1 = __builtin_amdgcn_alignbyte(c.y, c.x, 2), s2 = __builtin_amdgcn_alignbyte(c.z, c.y, 2),
                      s3 = __builtin_amdgcn_alignbyte(c.w, c.z, 2);
       const bf16x8 fm = __builtin_bit_cast(bf16x8, make_uint4(__builtin_amdgcn_alignbyte(c.x, pv, 2), s1, s2, s3));
