@@ -586,16 +586,8 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL((k_clip_apply<T>), dim3(1024, 1, B), dim3(256), 0, st2, nacc, n_state, RF, (const double*)clip_end,
                        1, rs);
     LD_LAUNCHED();
-    LD_TRY((pass<0, +1>(st, acc, w6, 3, inv())));
-    OpZRound<T, N, P3> orr{w6, CF, inv()};
-    const int nx = N / nslab;
-    for (int x0 = 0; x0 < N; x0 += nx) {
-      LD_TRY((pass<1, +1>(st, w6, w6, 3, inv(), x0, nx)));
-      LD_TRY((zrun<P3, 3, 3>(st, orr, -2, x0, nx)));
-      LD_TRY((pass<1, -1>(st, w6, w6, 3, (T)1, x0, nx)));
-    }
-    hipLaunchKernelGGL((k_x_proj<T, N, M>), dim3(NBLK_X, 1, B), dim3(M * N1), 0, st, (const C<T>*)w6, uk, CF,
-                       (const C<T>*)tw, maxpart, rs);
+    // reality round trip + projection + blow-up maxima in one spectral pass
+    hipLaunchKernelGGL((k_epi_proj<T, N>), dim3(NBLK_X, 1, B), dim3(256), 0, st, (const C<T>*)acc, uk, CF, maxpart, rs);
     LD_LAUNCHED();
     hipLaunchKernelGGL(k_max_final, dim3(1, 1, B), dim3(256), 0, st, (const double*)maxpart, NBLK_X,
                        100.0 * (double)N * N * N, flag, lastmax, rs);

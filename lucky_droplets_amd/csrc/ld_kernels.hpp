@@ -704,6 +704,66 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
   for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
 }
 
+// Step epilogue in spectral space (forced-dns-sm.py:597-610, 618): the
+// reality round trip u_hat <- rfftn(irfftn(u_hat)) and the projection, as one
+// pointwise pass.  The round trip leaves every mode with 0 < kz < N/2 as it is
+// and replaces the kz = 0 and kz = N/2 planes by their Hermitian part,
+// X(kx, ky) <- (X(kx, ky) + conj(X(-kx, -ky))) / 2 (the c2r along z keeps only
+// the real part of those planes' 2-D inverse transform); this kernel forms that
+// result directly, so it does not add the round trip's transform rounding.
+// Then u_hat += k (k . u_hat) mask / lap, and per-block maxima of Re(u_hat)
+// for the blow-up test.  Blocks take element runs grid-stride, four elements
+// of each field per thread in flight.
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_epi_proj(const C<T>* acc0, C<T>* uk0, size_t fs, double* maxpart0, size_t rs) {
+  const C<T>* acc = rsel1(acc0, rs);
+  C<T>* uk = rsel1(uk0, rs);
+  double* maxpart = rsel1(maxpart0, rs);
+  constexpr int Nf = N / 2 + 1, msk = N - 1;
+  constexpr uint32_t NE = (uint32_t)N * N * Nf;
+  constexpr int U = 4;
+  __shared__ double sh[256];
+  double mx = -INFINITY;
+  const uint32_t stride = gridDim.x * 256u;
+  for (uint32_t e0 = blockIdx.x * 256u + threadIdx.x; e0 < NE; e0 += U * stride) {
+    C<T> a[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t e = e0 + u * stride;
+      if (e < NE) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[u][c] = ld32(acc + c * fs, e);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t e = e0 + u * stride;
+      if (e >= NE) break;
+      const int iz = (int)(e % Nf), iy = (int)((e / Nf) & msk), ix = (int)(e / ((uint32_t)N * Nf));
+      if (iz == 0 || iz == N / 2) {
+        const uint32_t ep = ((uint32_t)((-ix) & msk) * N + (uint32_t)((-iy) & msk)) * Nf + iz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const C<T> p = ld32(acc + c * fs, ep);
+          a[u][c] = mkc<T>((a[u][c].x + p.x) * (T)0.5, (a[u][c].y - p.y) * (T)0.5);
+        }
+      }
+      const Mode<T> md = mode<T, N>(ix, iy, iz);
+      const C<T> d = mkc<T>(md.kx * a[u][0].x + md.ky * a[u][1].x + md.kz * a[u][2].x,
+                            md.kx * a[u][0].y + md.ky * a[u][1].y + md.kz * a[u][2].y);
+      const C<T> pk = cscale<T>(d, md.invlap);
+      const C<T> b0 = cadd<T>(a[u][0], cscale<T>(pk, md.kx));
+      const C<T> b1 = cadd<T>(a[u][1], cscale<T>(pk, md.ky));
+      const C<T> b2 = cadd<T>(a[u][2], cscale<T>(pk, md.kz));
+      st32(uk, e, b0); st32(uk + fs, e, b1); st32(uk + 2 * fs, e, b2);
+      mx = fmax(mx, fmax((double)b0.x, fmax((double)b1.x, (double)b2.x)));
+    }
+  }
+  const double bm = blockMax<256>(mx, sh);
+  if (threadIdx.x == 0) maxpart[blockIdx.x] = bm;
+}
+
 // Density RHS in spectral space (forced-dns-sm.py:325-330):
 //   kn_hat = (-i k.(nv)_hat*mask - fc_hat*mask) * mask, then x-inverse.
 // In: 4 fields (nv_x, nv_y, nv_z, fc) forward-transformed along z and y;
