@@ -49,6 +49,20 @@ template <typename G> __device__ __forceinline__ void sincosGT(double th, G& s, 
   }
 }
 
+// cosW4 with the sines and cosines taken in precision G (the angle in FP64).
+template <typename G>
+__device__ __forceinline__ int cosW4G(double x, double dx, double pi_2dx, G w[4]) {
+  const int i0 = (int)floor(x / dx) - 1;
+  G sn, cs;
+  sincosGT<G>((x - (double)i0 * dx) * pi_2dx, sn, cs);
+  const G q = (G)0.25;
+  w[0] = q * ((G)1 + cs);
+  w[1] = q * ((G)1 + sn);
+  w[2] = q * ((G)1 - cs);
+  w[3] = q * ((G)1 - sn);
+  return i0;
+}
+
 // Periodic wrap of one coordinate, x mod L with numpy's sign convention
 // (particles.py:269 `x[:, :d] %= L`).
 __device__ __forceinline__ double wrapL(double x, double L) {
@@ -453,7 +467,7 @@ template <int N> struct Brick {
   // z-runs of NBX bins split at tile boundaries into at most NZS key ranges
   static constexpr int NZS = (NBX + Bins<N>::TS - 1) / Bins<N>::TS + 1;
   static constexpr int NR = NZS * NBX * NBX;         // slot ranges
-  static constexpr int CH = 4 * NT;                  // candidates tested per round
+  static constexpr int CH = 2 * NT;                  // candidates tested per round (LDS fits 4 blocks per CU)
   static_assert(N % S == 0 && (NB * NB * NB) % 8 == 0, "brick grid");
 };
 
@@ -570,25 +584,28 @@ k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0
     const int n = cnt;
     for (int i = tid; i < n; i += NT) {
       const double4 d = dep[list[i]];
-      double wx[4], wy[4], wz[4];
-      const int i0 = cosW4(d.x, dx, pi_2dx, wx), j0 = cosW4(d.y, dx, pi_2dx, wy), k0 = cosW4(d.z, dx, pi_2dx, wz);
+      // stencil weights in the field precision (FP32 fields: FP32 sines and cosines)
+      using GT = std::conditional_t<sizeof(T) == 4, float, double>;
+      GT wx[4], wy[4], wz[4];
+      const int i0 = cosW4G<GT>(d.x, dx, pi_2dx, wx), j0 = cosW4G<GT>(d.y, dx, pi_2dx, wy),
+                k0 = cosW4G<GT>(d.z, dx, pi_2dx, wz);
       const int rx = ((i0 - c0x) % N + N) % N, ry = ((j0 - c0y) % N + N) % N, rz = ((k0 - c0z) % N + N) % N;
       const double W = d.w * P.inv_vol;
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         const int nx = (rx + a) % N;
         if (nx >= S) continue;
-        const double wa = W * wx[a];
+        const double wa = W * (double)wx[a];
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
           const int ny = (ry + bb) % N;
           if (ny >= S) continue;
-          const double wab = wa * wy[bb];
+          const double wab = wa * (double)wy[bb];
 #pragma unroll
           for (int cc = 0; cc < 4; ++cc) {
             const int nz = (rz + cc) % N;
             if (nz >= S) continue;
-            atomicAdd(&tile[(nx * S + ny) * S + nz], (unsigned long long)llrint(wab * wz[cc] * scale));
+            atomicAdd(&tile[(nx * S + ny) * S + nz], (unsigned long long)llrint(wab * (double)wz[cc] * scale));
           }
         }
       }
