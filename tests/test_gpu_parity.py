@@ -533,14 +533,16 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, float) - np.asarray(b, float)) / max(np.linalg.norm(b), 1e-300))
 
 
-def test_cnn_forward_and_gradients_match_oracle(gpu):
+@pytest.mark.parametrize("N, ch", [(8, (3, 16, 32, 3)), (16, (3, 32, 32, 16, 3)), (32, (3, 16, 32, 32, 3))])
+def test_cnn_forward_and_gradients_match_oracle(gpu, N, ch):
     """bf16 MFMA forward and backward against the CPU oracle with the same
     bfloat16 rounding points (FP32 vs exact accumulation left): 1e-2 / 3e-2
-    relative in norm."""
+    relative in norm.  N = 8 runs the 16x16x16 kernels; N >= 16 the 16x16x32
+    convolution (every channel pair 16/32, mirrored taps with the ReLU mask)
+    and the node-blocked weight gradient, z-line segments of 16 and 32 nodes."""
     import torch
     from lucky_droplets_amd.cnn import SubgridCNN
     from oracle import cnn as ocnn
-    N, ch = 8, (3, 16, 32, 3)
     net = SubgridCNN(N, ch)
     rng = np.random.default_rng(5)
     layers = [(rng.normal(0, np.sqrt(2.0 / (27 * ci)), (27, co, ci)).astype(np.float32),
