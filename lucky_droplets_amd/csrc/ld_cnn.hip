@@ -74,11 +74,14 @@ int conv2(ld_cnn* c, const bf16* X, const bf16* W, const float* b, const bf16* m
   int G = (nseg + 3) / 4;
   if (G > 2048) G = 2048;
   const dim3 g((unsigned)G), t(256);
-#define LD_NT_CASE(K)                                                                                          \
-  if (NT == K) {                                                                                               \
-    hipLaunchKernelGGL((k_conv2<CI, CO, FLIP, RELU, MASK, K>), g, t, 0, c->st, X, W, b, mask, Yb, Yf, N, nseg); \
-    LAUNCHED();                                                                                                \
-    return LD_OK;                                                                                              \
+#define LD_NT_CASE(K)                                                                                            \
+  if (NT == K) {                                                                                                 \
+    if constexpr (CI == 32)                                                                                      \
+      hipLaunchKernelGGL((k_conv3<CO, FLIP, RELU, MASK, K>), g, t, 0, c->st, X, W, b, mask, Yb, Yf, N, nseg);     \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_conv2<CI, CO, FLIP, RELU, MASK, K>), g, t, 0, c->st, X, W, b, mask, Yb, Yf, N, nseg); \
+    LAUNCHED();                                                                                                  \
+    return LD_OK;                                                                                                \
   }
   LD_NT_CASE(1) LD_NT_CASE(2) LD_NT_CASE(4) LD_NT_CASE(8)
 #undef LD_NT_CASE

@@ -188,6 +188,124 @@ k_conv2(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __r
   }
 }
 
+// CI = 32 form of k_conv2 that reuses input fragments across the dz taps: per
+// neighbour line (dx, dy) a wavefront loads its NT tiles once (plus the node
+// before and after the segment) and forms the dz = +1 and -1 operands by
+// shifting lanes within each 16-lane row (DPP row_shl:1 / row_shr:1, with the
+// row's end lane filled from the neighbouring tile by row_shr:15 / row_shl:15),
+// so one 16-byte load feeds 3 MT instructions instead of MT.
+template <unsigned CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t src) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, 0xf, 0xf, false);
+}
+// lanes of a 16-lane row take the fragment of the next node (dz = +1): lane i
+// <- lane i + 1 of `a`, lane 15 <- lane 0 of `next`
+__device__ __forceinline__ uint4 shift_up(const uint4& a, const uint4& next) {
+  uint4 r;
+  r.x = dpp_u<0x101>(dpp_u<0x11f>(0u, next.x), a.x);
+  r.y = dpp_u<0x101>(dpp_u<0x11f>(0u, next.y), a.y);
+  r.z = dpp_u<0x101>(dpp_u<0x11f>(0u, next.z), a.z);
+  r.w = dpp_u<0x101>(dpp_u<0x11f>(0u, next.w), a.w);
+  return r;
+}
+// dz = -1: lane i <- lane i - 1 of `a`, lane 0 <- lane 15 of `prev`
+__device__ __forceinline__ uint4 shift_down(const uint4& a, const uint4& prev) {
+  uint4 r;
+  r.x = dpp_u<0x111>(dpp_u<0x10f>(0u, prev.x), a.x);
+  r.y = dpp_u<0x111>(dpp_u<0x10f>(0u, prev.y), a.y);
+  r.z = dpp_u<0x111>(dpp_u<0x10f>(0u, prev.z), a.z);
+  r.w = dpp_u<0x111>(dpp_u<0x10f>(0u, prev.w), a.w);
+  return r;
+}
+
+template <int CO, bool FLIP, bool RELU, bool MASK, int NT>
+__global__ void __launch_bounds__(256)
+k_conv3(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __restrict__ bias,
+        const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N, int nseg) {
+  constexpr int CI = 32, MT = CO / 16;
+  constexpr int WN = 27 * CO * CI;
+  __shared__ __attribute__((aligned(16))) bf16 sW[WN];
+  for (int e = threadIdx.x; e < WN / 8; e += 256)
+    reinterpret_cast<uint4*>(sW)[e] = reinterpret_cast<const uint4*>(W)[e];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n16 = lane & 15, q = lane >> 4, ci0 = 8 * q;
+  const int G = gridDim.x;
+  const int lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  const int spl = N / (16 * NT);
+  const int msk = N - 1;
+  for (int seg = lb * 4 + wv; seg < nseg; seg += G * 4) {
+    const int line = seg / spl, z0 = (seg % spl) * 16 * NT;
+    const int x = line / N, y = line & msk;
+    f32x4 acc[NT][MT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) acc[t][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int dxy = 0; dxy < 9; ++dxy) {
+      const int ndx = dxy / 3 - 1, ndy = dxy % 3 - 1;     // neighbour line offset
+      const size_t row = ((size_t)((x + ndx) & msk) * N + ((y + ndy) & msk)) * N;
+      uint4 xc[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        xc[t] = *reinterpret_cast<const uint4*>(X + (row + ((z0 + 16 * t + n16) & msk)) * CI + ci0);
+      uint4 blo = make_uint4(0u, 0u, 0u, 0u), bhi = make_uint4(0u, 0u, 0u, 0u);
+      if (n16 == 15) blo = *reinterpret_cast<const uint4*>(X + (row + ((z0 - 1) & msk)) * CI + ci0);
+      if (n16 == 0) bhi = *reinterpret_cast<const uint4*>(X + (row + ((z0 + 16 * NT) & msk)) * CI + ci0);
+#pragma unroll
+      for (int ndz = -1; ndz <= 1; ++ndz) {
+        // weight tap of this neighbour offset (mirrored for the data gradient)
+        const int tap = FLIP ? (1 - ndx) * 9 + (1 - ndy) * 3 + (1 - ndz) : (ndx + 1) * 9 + (ndy + 1) * 3 + (ndz + 1);
+        int wofs = (tap * CO + n16) * CI + ci0;
+        asm volatile("" : "+v"(wofs));
+        bf16x8 wf[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+          wf[m] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(sW + wofs + m * 16 * CI));
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          uint4 v;
+          if (ndz == 0) v = xc[t];
+          else if (ndz > 0) v = shift_up(xc[t], t + 1 < NT ? xc[t + 1] : bhi);
+          else v = shift_down(xc[t], t > 0 ? xc[t - 1] : blo);
+          const bf16x8 xf = __builtin_bit_cast(bf16x8, v);
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[m], xf, acc[t][m], 0, 0, 0);
+        }
+      }
+    }
+    const size_t nbase = (size_t)line * N + z0 + n16;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const int co = m * 16 + 4 * q;
+        const size_t o = (nbase + 16 * t) * CO + co;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          v[i] = acc[t][m][i] + (bias ? bias[co + i] : 0.f);
+          if (RELU) v[i] = v[i] > 0.f ? v[i] : 0.f;
+        }
+        if (MASK) {
+          const uint2 mk = *reinterpret_cast<const uint2*>(mask + o);
+          const bf16 mb[4] = {(bf16)(mk.x & 0xffff), (bf16)(mk.x >> 16), (bf16)(mk.y & 0xffff), (bf16)(mk.y >> 16)};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = b2f(mb[i]) > 0.f ? v[i] : 0.f;
+        }
+        if (Yb) {
+          uint2 pk;
+          pk.x = (uint32_t)f2b(v[0]) | ((uint32_t)f2b(v[1]) << 16);
+          pk.y = (uint32_t)f2b(v[2]) | ((uint32_t)f2b(v[3]) << 16);
+          *reinterpret_cast<uint2*>(Yb + o) = pk;
+        }
+        if (Yf) *reinterpret_cast<float4*>(Yf + o) = make_float4(v[0], v[1], v[2], v[3]);
+      }
+  }
+}
+
 // Weight and bias gradients of one layer, per block over `chunk` nodes:
 //   part[blk][(tap CO + co) CI + ci] = sum_node dZ[node][co] X[nbr(node, tap)][ci]
 //   part[blk][27 CO CI + co]         = sum_node dZ[node][co]
