@@ -431,6 +431,56 @@ def test_ensemble_matches_single_runs(gpu, prec):
         d.close()
 
 
+def test_metric_workload_fp32_against_fp64_and_invariants(gpu):
+    """The metric workload itself (256^3 FP32 fields + 10^6 droplets), where the
+    CPU oracle is too slow: two fused steps in FP32 against the same steps of the
+    FP64 engine (validated against the oracle at small sizes) within the FP32
+    tolerance 1e-4, and size-independent properties of the FP32 result:
+    solenoidal u_hat, Hermitian kz = 0 and N/2 planes, n >= 0, finite rows in
+    input order."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = preset("c2")
+    rng = np.random.default_rng(cfg.seed)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, pid = ic.initial_droplets(cfg, rng)
+    del u
+    res = {}
+    for prec in ("f32", "f64"):
+        d = ForcedDNS(cfg.replace(precision=prec))
+        ukg, ng, cg = _t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64)
+        d.set_state(ukg, ng, cg)
+        d.advance(dt, 2)
+        d.get_state(ukg, ng, cg)
+        torch.cuda.synchronize()
+        assert not bool(d.status()[1])
+        d.close()
+        res[prec] = (ukg.to(torch.complex128), ng.to(torch.float64), cg)
+
+    def rel(a, b):
+        return float((a - b).abs().max() / b.abs().max())
+    u32, n32, c32 = res["f32"]
+    u64, n64, c64 = res["f64"]
+    assert rel(u32, u64) < 1e-4
+    assert rel(n32, n64) < 1e-4
+    assert rel(c32, c64) < 1e-4
+    assert bool(torch.isfinite(c32).all()) and bool(torch.isfinite(u32.abs()).all())
+    assert float(n32.min()) >= 0.0
+    N = cfg.N
+    k = torch.fft.fftfreq(N, 1.0 / N, dtype=torch.float64, device=gpu)
+    kz = torch.arange(N // 2 + 1, dtype=torch.float64, device=gpu)
+    div = (k[:, None, None] * u32[0] + k[None, :, None] * u32[1] + kz[None, None, :] * u32[2]).abs().max()
+    kmag = torch.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    assert float(div) < 1e-5 * float((kmag * u32.abs().amax(dim=0)).max())
+    for iz in (0, N // 2):
+        pl = u32[:, :, :, iz]
+        partner = torch.roll(torch.flip(pl, dims=(1, 2)), shifts=(1, 1), dims=(1, 2))
+        assert float((pl - partner.conj()).abs().max()) <= 1e-6 * float(u32.abs().max())
+
+
 def _c1_state(gpu, prec, npart, seed=11):
     import torch
     from lucky_droplets_amd import ic
