@@ -198,7 +198,7 @@ template <int N> struct Bins {
   static constexpr int BS = (N / 128 > 2) ? N / 128 : 2;
   static constexpr int B = N / BS;
   static constexpr int NB = B * B * B;
-  static constexpr int TC = (N >= 128) ? 16 : 8;    // tile side in cells
+  static constexpr int TC = (N >= 32) ? 16 : 8;     // tile side in cells
   static constexpr int TS = TC / BS;                 // tile side in bins
   static constexpr int NTt = B / TS;                 // tiles per side
   static_assert(TC % BS == 0 && B % TS == 0, "bin tiles");
