@@ -1,5 +1,6 @@
 // Calibration program for HBM traffic counters: a plain float4 copy of 1 GiB
-// (8192 x 256 threads, grid-stride), 3 warm-up and 20 timed launches.  Under
+// (one float4 per thread, 2^16 blocks of 256), 3 warm-up and 20 timed
+// launches.  Under
 // `rocprofv3 --pmc` the request-size counters of each dispatch must read back
 // 1 GiB read and 1 GiB written; the printed rate is the copy bandwidth of the
 // device (read + write bytes per second).
@@ -7,7 +8,8 @@
 #include <cstdio>
 
 __global__ void __launch_bounds__(256) copyk(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 #define CK(call)                                                                   \
@@ -30,11 +32,11 @@ int main() {
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  for (int it = 0; it < 3; ++it) hipLaunchKernelGGL(copyk, dim3(8192), dim3(256), 0, 0, a, b, n);
+  for (int it = 0; it < 3; ++it) hipLaunchKernelGGL(copyk, dim3((unsigned)(n / 256)), dim3(256), 0, 0, a, b, n);
   CK(hipGetLastError());
   const int reps = 20;
   CK(hipEventRecord(e0));
-  for (int it = 0; it < reps; ++it) hipLaunchKernelGGL(copyk, dim3(8192), dim3(256), 0, 0, a, b, n);
+  for (int it = 0; it < reps; ++it) hipLaunchKernelGGL(copyk, dim3((unsigned)(n / 256)), dim3(256), 0, 0, a, b, n);
   CK(hipEventRecord(e1));
   CK(hipEventSynchronize(e1));
   float ms = 0.f;
