@@ -422,20 +422,22 @@ k_cblock(const bf16* __restrict__ A, bf16* __restrict__ AB, size_t nodes) {
 // tap (dx, dy, dz) are eight consecutive z on the neighbour line, shifted by
 // dz: the lane loads the aligned 16 bytes at z plus one dword on each side
 // (periodic in z) and forms the dz = -1 and +1 operands with v_alignbyte.
-// Wavefront w of the 9 in a block owns the neighbour line (dx, dy) =
-// (w / 3 - 1, w % 3 - 1), i.e. taps 3 w .. 3 w + 2.  Block b sums nodes
+// Neighbour line l = 3 (dx + 1) + (dy + 1) carries taps 3 l .. 3 l + 2; a
+// wavefront owns LPW consecutive lines.  Block b sums nodes
 // [b chunk, (b + 1) chunk) in K-steps of 32 and writes its partial
 //   part[b][(tap CO + co) CI + ci],  part[b][27 CO CI + co] (bias, wavefront 0).
-template <int CI, int CO>
-__global__ void __launch_bounds__(576)
+// LPW neighbour lines per wavefront (9 / LPW wavefronts per block): LPW = 3
+// gives wavefront w the lines (w - 1, -1..1), so the dZ fragments are read by
+// three wavefronts instead of nine and each feeds 9 taps.
+template <int CI, int CO, int LPW>
+__global__ void __launch_bounds__(64 * 9 / LPW)
 k_wgrad2(const bf16* __restrict__ XB, const bf16* __restrict__ dZB, float* __restrict__ part, int N, int chunk) {
   constexpr int MT = CO / 16, NT = CI / 16;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int n16 = lane & 15, q = lane >> 4;
-  const int dx = w / 3 - 1, dy = w % 3 - 1, msk = N - 1;
-  f32x4 acc[3][MT][NT];
+  const int n16 = lane & 15, q = lane >> 4, msk = N - 1;
+  f32x4 acc[3 * LPW][MT][NT];
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
+  for (int d = 0; d < 3 * LPW; ++d)
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -450,7 +452,6 @@ k_wgrad2(const bf16* __restrict__ XB, const bf16* __restrict__ dZB, float* __res
     const int zq = (int)(nq & msk);
     const size_t line = nq / N;
     const int x = (int)(line / N), y = (int)(line & msk);
-    const size_t base = ((size_t)((x + dx) & msk) * N + ((y + dy) & msk)) * N;
     bf16x8 a[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
@@ -463,29 +464,35 @@ k_wgrad2(const bf16* __restrict__ XB, const bf16* __restrict__ dZB, float* __res
       }
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int ci = t * 16 + n16;
-      const size_t np = base + ((zq - 2) & msk), nn = base + ((zq + 8) & msk);
-      const uint4 c = *reinterpret_cast<const uint4*>(XB + (((base + zq) >> 3) * CI + ci) * 8);
-      const uint32_t pv = *reinterpret_cast<const uint32_t*>(XB + ((np >> 3) * CI + ci) * 8 + (np & 7));
-      const uint32_t nv = *reinterpret_cast<const uint32_t*>(XB + ((nn >> 3) * CI + ci) * 8 + (nn & 7));
-      const uint32_t s1 = __builtin_amdgcn_alignbyte(c.y, c.x, 2), s2 = __builtin_amdgcn_alignbyte(c.z, c.y, 2),
-                     s3 = __builtin_amdgcn_alignbyte(c.w, c.z, 2);
-      const bf16x8 fm = __builtin_bit_cast(bf16x8, make_uint4(__builtin_amdgcn_alignbyte(c.x, pv, 2), s1, s2, s3));
-      const bf16x8 f0 = __builtin_bit_cast(bf16x8, c);
-      const bf16x8 fp = __builtin_bit_cast(bf16x8, make_uint4(s1, s2, s3, __builtin_amdgcn_alignbyte(nv, c.w, 2)));
+    for (int li = 0; li < LPW; ++li) {
+      const int wl = w * LPW + li;                 // neighbour line index 0..8
+      const int dx = wl / 3 - 1, dy = wl % 3 - 1;
+      const size_t base = ((size_t)((x + dx) & msk) * N + ((y + dy) & msk)) * N;
 #pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        acc[0][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], fm, acc[0][m][t], 0, 0, 0);
-        acc[1][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], f0, acc[1][m][t], 0, 0, 0);
-        acc[2][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], fp, acc[2][m][t], 0, 0, 0);
+      for (int t = 0; t < NT; ++t) {
+        const int ci = t * 16 + n16;
+        const size_t np = base + ((zq - 2) & msk), nn = base + ((zq + 8) & msk);
+        const uint4 c = *reinterpret_cast<const uint4*>(XB + (((base + zq) >> 3) * CI + ci) * 8);
+        const uint32_t pv = *reinterpret_cast<const uint32_t*>(XB + ((np >> 3) * CI + ci) * 8 + (np & 7));
+        const uint32_t nv = *reinterpret_cast<const uint32_t*>(XB + ((nn >> 3) * CI + ci) * 8 + (nn & 7));
+        const uint32_t s1 = __builtin_amdgcn_alignbyte(c.y, c.x, 2), s2 = __builtin_amdgcn_alignbyte(c.z, c.y, 2),
+                       s3 = __builtin_amdgcn_alignbyte(c.w, c.z, 2);
+        const bf16x8 fm = __builtin_bit_cast(bf16x8, make_uint4(__builtin_amdgcn_alignbyte(c.x, pv, 2), s1, s2, s3));
+        const bf16x8 f0 = __builtin_bit_cast(bf16x8, c);
+        const bf16x8 fp = __builtin_bit_cast(bf16x8, make_uint4(s1, s2, s3, __builtin_amdgcn_alignbyte(nv, c.w, 2)));
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          acc[3 * li][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], fm, acc[3 * li][m][t], 0, 0, 0);
+          acc[3 * li + 1][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], f0, acc[3 * li + 1][m][t], 0, 0, 0);
+          acc[3 * li + 2][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], fp, acc[3 * li + 2][m][t], 0, 0, 0);
+        }
       }
     }
   }
   float* out = part + (size_t)blockIdx.x * (27 * CO * CI + CO);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int tap = 3 * w + d;
+  for (int d = 0; d < 3 * LPW; ++d) {
+    const int tap = 3 * (w * LPW) + d;
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
