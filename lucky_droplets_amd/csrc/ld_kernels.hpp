@@ -712,8 +712,8 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
 // the real part of those planes' 2-D inverse transform); this kernel forms that
 // result directly, so it does not add the round trip's transform rounding.
 // Then u_hat += k (k . u_hat) mask / lap, and per-block maxima of Re(u_hat)
-// for the blow-up test.  Blocks take element runs grid-stride, four elements
-// of each field per thread in flight.
+// for the blow-up test.  Blocks take runs of entry pairs grid-stride, two
+// pairs of each field per thread in flight.
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
 k_epi_proj(const C<T>* acc0, C<T>* uk0, size_t fs, double* maxpart0, size_t rs) {
@@ -721,43 +721,52 @@ k_epi_proj(const C<T>* acc0, C<T>* uk0, size_t fs, double* maxpart0, size_t rs) 
   C<T>* uk = rsel1(uk0, rs);
   double* maxpart = rsel1(maxpart0, rs);
   constexpr int Nf = N / 2 + 1, msk = N - 1;
-  constexpr uint32_t NE = (uint32_t)N * N * Nf;
-  constexpr int U = 4;
+  constexpr uint32_t NE2 = (uint32_t)N * N * Nf / 2;     // entry pairs (N * N is even)
+  constexpr int U = 2;
   __shared__ double sh[256];
   double mx = -INFINITY;
   const uint32_t stride = gridDim.x * 256u;
-  for (uint32_t e0 = blockIdx.x * 256u + threadIdx.x; e0 < NE; e0 += U * stride) {
-    C<T> a[U][3];
+  // a thread takes entry pairs (2 e2, 2 e2 + 1): 16-byte FP32 accesses
+  for (uint32_t e0 = blockIdx.x * 256u + threadIdx.x; e0 < NE2; e0 += U * stride) {
+    C2<T> a[U][3];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t e = e0 + u * stride;
-      if (e < NE) {
+      const uint32_t e2 = e0 + u * stride;
+      if (e2 < NE2) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) a[u][c] = ld32(acc + c * fs, e);
+        for (int c = 0; c < 3; ++c) a[u][c] = ld32(reinterpret_cast<const C2<T>*>(acc + c * fs), e2);
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t e = e0 + u * stride;
-      if (e >= NE) break;
-      const int iz = (int)(e % Nf), iy = (int)((e / Nf) & msk), ix = (int)(e / ((uint32_t)N * Nf));
-      if (iz == 0 || iz == N / 2) {
-        const uint32_t ep = ((uint32_t)((-ix) & msk) * N + (uint32_t)((-iy) & msk)) * Nf + iz;
+      const uint32_t e2 = e0 + u * stride;
+      if (e2 >= NE2) break;
+      C2<T> o[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const C<T> p = ld32(acc + c * fs, ep);
-          a[u][c] = mkc<T>((a[u][c].x + p.x) * (T)0.5, (a[u][c].y - p.y) * (T)0.5);
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t e = 2 * e2 + h;
+        const int iz = (int)(e % Nf), iy = (int)((e / Nf) & msk), ix = (int)(e / ((uint32_t)N * Nf));
+        C<T> v[3] = {h ? a[u][0].b : a[u][0].a, h ? a[u][1].b : a[u][1].a, h ? a[u][2].b : a[u][2].a};
+        if (iz == 0 || iz == N / 2) {
+          const uint32_t ep = ((uint32_t)((-ix) & msk) * N + (uint32_t)((-iy) & msk)) * Nf + iz;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const C<T> p = ld32(acc + c * fs, ep);
+            v[c] = mkc<T>((v[c].x + p.x) * (T)0.5, (v[c].y - p.y) * (T)0.5);
+          }
         }
+        const Mode<T> md = mode<T, N>(ix, iy, iz);
+        const C<T> d = mkc<T>(md.kx * v[0].x + md.ky * v[1].x + md.kz * v[2].x,
+                              md.kx * v[0].y + md.ky * v[1].y + md.kz * v[2].y);
+        const C<T> pk = cscale<T>(d, md.invlap);
+        const C<T> b0 = cadd<T>(v[0], cscale<T>(pk, md.kx));
+        const C<T> b1 = cadd<T>(v[1], cscale<T>(pk, md.ky));
+        const C<T> b2 = cadd<T>(v[2], cscale<T>(pk, md.kz));
+        if (h) { o[0].b = b0; o[1].b = b1; o[2].b = b2; } else { o[0].a = b0; o[1].a = b1; o[2].a = b2; }
+        mx = fmax(mx, fmax((double)b0.x, fmax((double)b1.x, (double)b2.x)));
       }
-      const Mode<T> md = mode<T, N>(ix, iy, iz);
-      const C<T> d = mkc<T>(md.kx * a[u][0].x + md.ky * a[u][1].x + md.kz * a[u][2].x,
-                            md.kx * a[u][0].y + md.ky * a[u][1].y + md.kz * a[u][2].y);
-      const C<T> pk = cscale<T>(d, md.invlap);
-      const C<T> b0 = cadd<T>(a[u][0], cscale<T>(pk, md.kx));
-      const C<T> b1 = cadd<T>(a[u][1], cscale<T>(pk, md.ky));
-      const C<T> b2 = cadd<T>(a[u][2], cscale<T>(pk, md.kz));
-      st32(uk, e, b0); st32(uk + fs, e, b1); st32(uk + 2 * fs, e, b2);
-      mx = fmax(mx, fmax((double)b0.x, fmax((double)b1.x, (double)b2.x)));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) st32(reinterpret_cast<C2<T>*>(uk + c * fs), e2, o[c]);
     }
   }
   const double bm = blockMax<256>(mx, sh);

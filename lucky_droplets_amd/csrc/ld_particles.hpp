@@ -590,7 +590,10 @@ k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0
       const int i0 = cosW4G<GT>(d.x, dx, pi_2dx, wx), j0 = cosW4G<GT>(d.y, dx, pi_2dx, wy),
                 k0 = cosW4G<GT>(d.z, dx, pi_2dx, wz);
       const int rx = ((i0 - c0x) % N + N) % N, ry = ((j0 - c0y) % N + N) % N, rz = ((k0 - c0z) % N + N) % N;
-      const double W = d.w * P.inv_vol;
+      // scale is a power of two, so W * scale is exact and the products below
+      // round as (W wx wy wz) scale would
+      const double W = d.w * P.inv_vol * scale;
+      const double wzd[4] = {(double)wz[0], (double)wz[1], (double)wz[2], (double)wz[3]};
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         const int nx = (rx + a) % N;
@@ -605,7 +608,7 @@ k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0
           for (int cc = 0; cc < 4; ++cc) {
             const int nz = (rz + cc) % N;
             if (nz >= S) continue;
-            atomicAdd(&tile[(nx * S + ny) * S + nz], (unsigned long long)llrint(wab * (double)wz[cc] * scale));
+            atomicAdd(&tile[(nx * S + ny) * S + nz], (unsigned long long)llrint(wab * wzd[cc]));
           }
         }
       }
