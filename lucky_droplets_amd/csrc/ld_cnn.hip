@@ -31,6 +31,7 @@ struct ld_cnn {
   int nblk_w = 0, chunk = 0;
   int nblk_w2 = 0, chunk2 = 0;              // k_wgrad2 blocks and nodes per block (N >= 16)
   bf16 *xt = nullptr, *dzt = nullptr;       // node-blocked channel-major copies for the weight gradient
+  bf16* actB[LD_CNN_MAX_LAYERS] = {};       // node-blocked copies of the hidden activations (N >= 16)
   int adam_t = 0;
   bool packed = false;
   ~ld_cnn() {
@@ -67,7 +68,8 @@ namespace {
 // gfx950 double-rate form: NT node tiles of 16 per wavefront (a z-line
 // segment of 16 NT nodes), persistent blocks of 4 wavefronts.
 template <int CI, int CO, bool FLIP, bool RELU, bool MASK>
-int conv2(ld_cnn* c, const bf16* X, const bf16* W, const float* b, const bf16* mask, bf16* Yb, float* Yf) {
+int conv2(ld_cnn* c, const bf16* X, const bf16* W, const float* b, const bf16* mask, bf16* Yb, float* Yf,
+          bf16* Yblk) {
   const int N = c->N;
   const int NT = N >= 128 ? 8 : N / 16;
   const int nseg = (int)(c->nodes / (16 * NT));
@@ -77,9 +79,10 @@ int conv2(ld_cnn* c, const bf16* X, const bf16* W, const float* b, const bf16* m
 #define LD_NT_CASE(K)                                                                                            \
   if (NT == K) {                                                                                                 \
     if constexpr (CI == 32)                                                                                      \
-      hipLaunchKernelGGL((k_conv3<CO, FLIP, RELU, MASK, K>), g, t, 0, c->st, X, W, b, mask, Yb, Yf, N, nseg);     \
+      hipLaunchKernelGGL((k_conv3<CO, FLIP, RELU, MASK, K>), g, t, 0, c->st, X, W, b, mask, Yb, Yf, N, nseg, Yblk); \
     else                                                                                                         \
-      hipLaunchKernelGGL((k_conv2<CI, CO, FLIP, RELU, MASK, K>), g, t, 0, c->st, X, W, b, mask, Yb, Yf, N, nseg); \
+      hipLaunchKernelGGL((k_conv2<CI, CO, FLIP, RELU, MASK, K>), g, t, 0, c->st, X, W, b, mask, Yb, Yf, N, nseg, \
+                         Yblk);                                                                                  \
     LAUNCHED();                                                                                                  \
     return LD_OK;                                                                                                \
   }
@@ -89,14 +92,16 @@ int conv2(ld_cnn* c, const bf16* X, const bf16* W, const float* b, const bf16* m
   return LD_ERR_ARG;
 }
 
+// Yblk (N >= 16 only; may be null): a second copy of Yb in the node-blocked
+// layout the weight gradient reads.
 template <bool FLIP, bool RELU, bool MASK>
 int conv(ld_cnn* c, int CI, int CO, const bf16* X, const bf16* W, const float* b, const bf16* mask, bf16* Yb,
-         float* Yf) {
+         float* Yf, bf16* Yblk = nullptr) {
   if (c->N >= 16) {
-    if (CI == 16 && CO == 16) return conv2<16, 16, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf);
-    if (CI == 16 && CO == 32) return conv2<16, 32, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf);
-    if (CI == 32 && CO == 16) return conv2<32, 16, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf);
-    if (CI == 32 && CO == 32) return conv2<32, 32, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf);
+    if (CI == 16 && CO == 16) return conv2<16, 16, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf, Yblk);
+    if (CI == 16 && CO == 32) return conv2<16, 32, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf, Yblk);
+    if (CI == 32 && CO == 16) return conv2<32, 16, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf, Yblk);
+    if (CI == 32 && CO == 32) return conv2<32, 32, FLIP, RELU, MASK>(c, X, W, b, mask, Yb, Yf, Yblk);
   }
   const dim3 g((unsigned)(c->nodes / 64)), t(256);
 #define LD_CONV_CASE(A, B)                                                                                    \
@@ -119,17 +124,19 @@ template <int C> int cmajor(ld_cnn* c, const bf16* A, bf16* AT) {
 }
 
 // Weight and bias gradients of one layer into c->part (per-block partials);
-// returns the number of partial blocks in *nblk.
-int wgrad(ld_cnn* c, int CI, int CO, const bf16* X, const bf16* dZ, int* nblk) {
+// returns the number of partial blocks in *nblk.  XB / dZB: node-blocked
+// copies of X / dZ when a producing kernel already wrote them (N >= 16), else
+// null and they are made here.
+int wgrad(ld_cnn* c, int CI, int CO, const bf16* X, const bf16* dZ, const bf16* XB, const bf16* dZB, int* nblk) {
   if (c->N >= 16) {
-    CK(CI == 16 ? cmajor<16>(c, X, c->xt) : cmajor<32>(c, X, c->xt));
-    CK(CO == 16 ? cmajor<16>(c, dZ, c->dzt) : cmajor<32>(c, dZ, c->dzt));
+    if (!XB) { CK(CI == 16 ? cmajor<16>(c, X, c->xt) : cmajor<32>(c, X, c->xt)); XB = c->xt; }
+    if (!dZB) { CK(CO == 16 ? cmajor<16>(c, dZ, c->dzt) : cmajor<32>(c, dZ, c->dzt)); dZB = c->dzt; }
     // one neighbour line per wavefront (three measured slower: 396 -> 411 us at 128^3, 32 -> 32)
     constexpr int LPW = 1;
     const dim3 g((unsigned)c->nblk_w2), t(64 * 9 / LPW);
 #define LD_WG2_CASE(A, B)                                                                                  \
     if (CI == A && CO == B) {                                                                              \
-      hipLaunchKernelGGL((k_wgrad2<A, B, LPW>), g, t, 0, c->st, c->xt, c->dzt, c->part, c->N, c->chunk2);    \
+      hipLaunchKernelGGL((k_wgrad2<A, B, LPW>), g, t, 0, c->st, XB, dZB, c->part, c->N, c->chunk2);          \
       LAUNCHED();                                                                                          \
       *nblk = c->nblk_w2;                                                                                  \
       return LD_OK;                                                                                        \
@@ -164,8 +171,10 @@ int pack_weights(ld_cnn* c) {
 }
 
 // Forward pass from the FP32 input rows x[node][cin]; keeps every layer's
-// input for the backward pass; FP32 output in c->yp.
-int forward(ld_cnn* c, const float* x) {
+// input for the backward pass; FP32 output in c->yp.  for_grad: the hidden
+// layers also write node-blocked copies of their outputs (c->actB, N >= 16)
+// for the weight gradient.
+int forward(ld_cnn* c, const float* x, bool for_grad) {
   if (!c->packed) CK(pack_weights(c));
   const size_t gp = (c->nodes + 255) / 256;
   hipLaunchKernelGGL(k_pack_input, dim3((unsigned)(gp < (1u << 20) ? gp : (1u << 20))), dim3(256), 0, c->st, x, c->cin,
@@ -174,7 +183,8 @@ int forward(ld_cnn* c, const float* x) {
   for (int l = 0; l < c->L; ++l) {
     const float* b = c->params + c->off[l] + 27LL * c->ch[l + 1] * c->ch[l];
     if (l + 1 < c->L)
-      CK((conv<false, true, false>(c, c->ch[l], c->ch[l + 1], c->act[l], c->Wb[l], b, nullptr, c->act[l + 1], nullptr)));
+      CK((conv<false, true, false>(c, c->ch[l], c->ch[l + 1], c->act[l], c->Wb[l], b, nullptr, c->act[l + 1], nullptr,
+                                   for_grad ? c->actB[l + 1] : nullptr)));
     else
       CK((conv<false, false, false>(c, c->ch[l], c->ch[l + 1], c->act[l], c->Wb[l], b, nullptr, nullptr, c->yp)));
   }
@@ -189,10 +199,11 @@ int backward(ld_cnn* c, const float* target, double* loss) {
                      c->dz[0], c->mpart);
   LAUNCHED();
   int cur = 0;
+  bool dzb = false;        // c->dzt holds the node-blocked copy of dz[cur]
   for (int l = c->L - 1; l >= 0; --l) {
     const int CI = c->ch[l], CO = c->ch[l + 1];
     int nblk = 0;
-    CK(wgrad(c, CI, CO, c->act[l], c->dz[cur], &nblk));
+    CK(wgrad(c, CI, CO, c->act[l], c->dz[cur], l > 0 ? c->actB[l] : nullptr, dzb ? c->dzt : nullptr, &nblk));
     const int n = 27 * CO * CI + CO;
     hipLaunchKernelGGL(k_reduce_parts, dim3((n + 255) / 256), dim3(256), 0, c->st, (const float*)c->part, nblk, n,
                        c->grads + c->off[l]);
@@ -200,7 +211,9 @@ int backward(ld_cnn* c, const float* target, double* loss) {
     if (l > 0) {
       // data gradient through the convolution (transposed weights, mirrored taps)
       // and the ReLU of layer l - 1 (whose output is act[l])
-      CK((conv<true, false, true>(c, CO, CI, c->dz[cur], c->Wt[l], nullptr, c->act[l], c->dz[cur ^ 1], nullptr)));
+      CK((conv<true, false, true>(c, CO, CI, c->dz[cur], c->Wt[l], nullptr, c->act[l], c->dz[cur ^ 1], nullptr,
+                                  c->N >= 16 ? c->dzt : nullptr)));
+      dzb = c->N >= 16;
       cur ^= 1;
     }
   }
@@ -275,6 +288,7 @@ int ld_cnn_create(const ld_cnn_params* p, ld_cnn** out) {
   if (rc == LD_OK) rc = c->alloc(c->part, (size_t)(c->nblk_w > c->nblk_w2 ? c->nblk_w : c->nblk_w2) * maxlayer);
   if (rc == LD_OK && c->N >= 16) rc = c->alloc(c->xt, c->nodes * 32);
   if (rc == LD_OK && c->N >= 16) rc = c->alloc(c->dzt, c->nodes * 32);
+  for (int l = 1; l < c->L && rc == LD_OK && c->N >= 16; ++l) rc = c->alloc(c->actB[l], c->nodes * c->ch[l]);
   if (rc == LD_OK) rc = c->alloc(c->mpart, LD_CNN_MSE_BLOCKS);
   if (rc == LD_OK && hipMemset(c->params, 0, np * sizeof(float)) != hipSuccess) rc = LD_ERR_HIP;
   if (rc == LD_OK && hipMemset(c->m, 0, np * sizeof(float)) != hipSuccess) rc = LD_ERR_HIP;
@@ -322,7 +336,7 @@ int ld_cnn_get_params(ld_cnn* c, float* params) {
 
 int ld_cnn_forward(ld_cnn* c, const float* x, float* y) {
   if (!c || !x || !y) { set_error("null argument"); return LD_ERR_ARG; }
-  CK(forward(c, x));
+  CK(forward(c, x, false));
   hipLaunchKernelGGL(k_unpack_output, dim3(1024), dim3(256), 0, c->st, (const float*)c->yp, c->ch[c->L], c->cout,
                      c->nodes, y);
   LAUNCHED();
@@ -331,7 +345,7 @@ int ld_cnn_forward(ld_cnn* c, const float* x, float* y) {
 
 int ld_cnn_grad(ld_cnn* c, const float* x, const float* target, float* grads, double* loss) {
   if (!c || !x || !target) { set_error("null argument"); return LD_ERR_ARG; }
-  CK(forward(c, x));
+  CK(forward(c, x, true));
   CK(backward(c, target, loss));
   if (grads) HK(hipMemcpyAsync(grads, c->grads, c->nparams * sizeof(float), hipMemcpyDeviceToDevice, c->st));
   return LD_OK;
@@ -340,7 +354,7 @@ int ld_cnn_grad(ld_cnn* c, const float* x, const float* target, float* grads, do
 int ld_cnn_train_step(ld_cnn* c, const float* x, const float* target, float lr, float beta1, float beta2, float eps,
                       double* loss) {
   if (!c || !x || !target) { set_error("null argument"); return LD_ERR_ARG; }
-  CK(forward(c, x));
+  CK(forward(c, x, true));
   CK(backward(c, target, loss));
   c->adam_t += 1;
   hipLaunchKernelGGL(k_adam, dim3((unsigned)((c->nparams + 255) / 256)), dim3(256), 0, c->st, c->params,

@@ -102,15 +102,39 @@ k_conv(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __re
 // Needs N >= 16 * NT and N a multiple of 16 * NT.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
+// Writes a wavefront's 16-node x 16-channel output tile (lane l: node n0 +
+// l % 16, channels c0 + 4 (l / 16) + i, i < 4, in v) to the node-blocked
+// channel-major layout of k_cblock through 512 bytes of LDS owned by the
+// wavefront: 32 runs of 8 nodes x 1 channel, one 16-byte store each.  n0 is a
+// multiple of 16.  The LDS writes and reads are issued by one wavefront, whose
+// LDS operations complete in order.
+template <int CO>
+__device__ __forceinline__ void store_blocked(bf16* __restrict__ Yblk, bf16* sT, size_t n0, int c0, const uint32_t pk[2]) {
+  const int lane = threadIdx.x & 63, n16 = lane & 15, q = lane >> 4;
+  sT[(4 * q + 0) * 16 + n16] = (bf16)(pk[0] & 0xffff);
+  sT[(4 * q + 1) * 16 + n16] = (bf16)(pk[0] >> 16);
+  sT[(4 * q + 2) * 16 + n16] = (bf16)(pk[1] & 0xffff);
+  sT[(4 * q + 3) * 16 + n16] = (bf16)(pk[1] >> 16);
+  __builtin_amdgcn_wave_barrier();
+  if (lane < 32) {
+    const int cl = lane >> 1, half = lane & 1;
+    const uint4 v = *reinterpret_cast<const uint4*>(sT + cl * 16 + half * 8);
+    *reinterpret_cast<uint4*>(Yblk + (((n0 >> 3) + half) * CO + c0 + cl) * 8) = v;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <int CI, int CO, bool FLIP, bool RELU, bool MASK, int NT>
 __global__ void __launch_bounds__(256)
 k_conv2(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __restrict__ bias,
-        const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N, int nseg) {
+        const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N, int nseg,
+        bf16* __restrict__ Yblk) {
   static_assert(CI % 16 == 0 && CO % 16 == 0, "channels: multiples of 16");
   constexpr int MT = CO / 16;
   constexpr int KS = (27 * CI + 31) / 32;
   constexpr int WN = 27 * CO * CI;
   __shared__ __attribute__((aligned(16))) bf16 sW[WN];
+  __shared__ __attribute__((aligned(16))) bf16 sT[4][256];   // per-wavefront tile for the blocked copy
   for (int e = threadIdx.x; e < WN / 8; e += 256)
     reinterpret_cast<uint4*>(sW)[e] = reinterpret_cast<const uint4*>(W)[e];
   __syncthreads();
@@ -177,13 +201,12 @@ k_conv2(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __r
 #pragma unroll
           for (int i = 0; i < 4; ++i) v[i] = b2f(mb[i]) > 0.f ? v[i] : 0.f;
         }
-        if (Yb) {
-          uint2 pk;
-          pk.x = (uint32_t)f2b(v[0]) | ((uint32_t)f2b(v[1]) << 16);
-          pk.y = (uint32_t)f2b(v[2]) | ((uint32_t)f2b(v[3]) << 16);
-          *reinterpret_cast<uint2*>(Yb + o) = pk;
-        }
+        uint32_t pk[2];
+        pk[0] = (uint32_t)f2b(v[0]) | ((uint32_t)f2b(v[1]) << 16);
+        pk[1] = (uint32_t)f2b(v[2]) | ((uint32_t)f2b(v[3]) << 16);
+        if (Yb) *reinterpret_cast<uint2*>(Yb + o) = make_uint2(pk[0], pk[1]);
         if (Yf) *reinterpret_cast<float4*>(Yf + o) = make_float4(v[0], v[1], v[2], v[3]);
+        if (Yblk) store_blocked<CO>(Yblk, sT[wv], nbase - n16 + 16 * t, m * 16, pk);
       }
   }
 }
@@ -221,10 +244,12 @@ __device__ __forceinline__ uint4 shift_down(const uint4& a, const uint4& prev) {
 template <int CO, bool FLIP, bool RELU, bool MASK, int NT>
 __global__ void __launch_bounds__(256)
 k_conv3(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __restrict__ bias,
-        const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N, int nseg) {
+        const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N, int nseg,
+        bf16* __restrict__ Yblk) {
   constexpr int CI = 32, MT = CO / 16;
   constexpr int WN = 27 * CO * CI;
   __shared__ __attribute__((aligned(16))) bf16 sW[WN];
+  __shared__ __attribute__((aligned(16))) bf16 sT[4][256];   // per-wavefront tile for the blocked copy
   for (int e = threadIdx.x; e < WN / 8; e += 256)
     reinterpret_cast<uint4*>(sW)[e] = reinterpret_cast<const uint4*>(W)[e];
   __syncthreads();
@@ -295,13 +320,12 @@ k_conv3(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __r
 #pragma unroll
           for (int i = 0; i < 4; ++i) v[i] = b2f(mb[i]) > 0.f ? v[i] : 0.f;
         }
-        if (Yb) {
-          uint2 pk;
-          pk.x = (uint32_t)f2b(v[0]) | ((uint32_t)f2b(v[1]) << 16);
-          pk.y = (uint32_t)f2b(v[2]) | ((uint32_t)f2b(v[3]) << 16);
-          *reinterpret_cast<uint2*>(Yb + o) = pk;
-        }
+        uint32_t pk[2];
+        pk[0] = (uint32_t)f2b(v[0]) | ((uint32_t)f2b(v[1]) << 16);
+        pk[1] = (uint32_t)f2b(v[2]) | ((uint32_t)f2b(v[3]) << 16);
+        if (Yb) *reinterpret_cast<uint2*>(Yb + o) = make_uint2(pk[0], pk[1]);
         if (Yf) *reinterpret_cast<float4*>(Yf + o) = make_float4(v[0], v[1], v[2], v[3]);
+        if (Yblk) store_blocked<CO>(Yblk, sT[wv], nbase - n16 + 16 * t, m * 16, pk);
       }
   }
 }
