@@ -352,24 +352,6 @@ struct OpZFwd {
     }
   }
 };
-// c2r followed by r2c of 3 fields (the reality round trip of
-// forced-dns-sm.py:597-603), in place.
-template <typename T, int N, int P>
-struct OpZRound {
-  C<T>* buf; size_t fs; T scale;
-  __device__ OpZRound shifted(size_t rs) const { OpZRound o = *this; o.buf = rsel1(buf, rs); return o; }
-  __device__ const C<T>* specIn(int f) const { return buf + f * fs; }
-  __device__ C<T>* specOut(int f) const { return buf + f * fs; }
-  __device__ void physical(const ZTile<T, N, P>& t, int) const {
-    using Q = Quads<ZThreads<N, P, 3, 3>::v, 3 * P * N / 4>;
-#pragma unroll
-    for (int it = 0; it < Q::IT; ++it) {
-      if (!Q::ok(it)) break;
-      const int e4 = 4 * Q::e(it), f = e4 / (P * N), rem = e4 % (P * N), p = rem / N, j = rem % N;
-      t.re4(f, p, j) = scale4(t.re4(f, p, j), scale);
-    }
-  }
-};
 
 // Velocity and vorticity to physical space, u stored, then the rotational
 // nonlinear term u x omega transformed along z (forced-dns-sm.py:291-301, and
@@ -829,50 +811,6 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs) {
   }
 }
 
-// Last pass of the reality round trip + solenoidal projection
-// (forced-dns-sm.py:601-610) + blow-up statistic max Re(u_hat) (:618).
-template <typename T, int N, int M>
-__global__ void __launch_bounds__(M * Plan<N>::N1)
-k_x_proj(const C<T>* w0, C<T>* uk0, size_t fs, const C<T>* __restrict__ tw, double* maxpart0, size_t rs) {
-  const C<T>* w = rsel1(w0, rs);
-  C<T>* uk = rsel1(uk0, rs);
-  double* maxpart = rsel1(maxpart0, rs);
-  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf;
-  constexpr int NT = M * N1;
-  __shared__ C<T> lds[M * (N + 1)];
-  __shared__ double sh[NT];
-  const int m = threadIdx.x % M, n1 = threadIdx.x / M;
-  const int q = blockIdx.x * M + m;
-  const int iy = q / Nf, iz = q % Nf;
-  C<T>* L = lds + m * (N + 1);
-  C<T> a0[N2], a1[N2], a2[N2];
-#pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) {
-    const uint32_t o = (uint32_t)(q + posIn<N>(n1, n2) * js);
-    a0[n2] = ld32(w, o); a1[n2] = ld32(w + fs, o); a2[n2] = ld32(w + 2 * fs, o);
-  }
-  lineFwd<T, N>(a0, L, n1, true, tw);
-  lineFwd<T, N>(a1, L, n1, true, tw);
-  lineFwd<T, N>(a2, L, n1, true, tw);
-  double mx = -INFINITY;
-#pragma unroll
-  for (int r = 0; r < N2; ++r) {
-    const int ix = posOut<N>(n1, r);
-    const Mode<T> md = mode<T, N>(ix, iy, iz);
-    const C<T> d = mkc<T>(md.kx * a0[r].x + md.ky * a1[r].x + md.kz * a2[r].x,
-                          md.kx * a0[r].y + md.ky * a1[r].y + md.kz * a2[r].y);
-    const C<T> pk = cscale<T>(d, md.invlap);
-    const C<T> b0 = cadd<T>(a0[r], cscale<T>(pk, md.kx));
-    const C<T> b1 = cadd<T>(a1[r], cscale<T>(pk, md.ky));
-    const C<T> b2 = cadd<T>(a2[r], cscale<T>(pk, md.kz));
-    const uint32_t o = (uint32_t)(q + ix * js);
-    st32(uk, o, b0); st32(uk + fs, o, b1); st32(uk + 2 * fs, o, b2);
-    mx = fmax(mx, fmax((double)b0.x, fmax((double)b1.x, (double)b2.x)));
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  const double bm = blockMax<NT>(mx, sh);
-  if (threadIdx.x == 0) maxpart[blockIdx.x] = bm;
-}
 
 // ======================================================== small kernels
 // Shell energies of the forced shells and the constant-power factors
