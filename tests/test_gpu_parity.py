@@ -388,6 +388,35 @@ def test_advance_bitwise_reproducible(gpu, prec):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+@pytest.mark.parametrize("N", [64, 128])
+def test_density_chain_order_keeps_results(gpu, N, monkeypatch):
+    """Queueing each stage's density chain after the next flow x pass
+    (LD_LATE_DENSITY=1, the default from N = 128) or right after its own flow
+    stage (0) runs the same kernels on the same data: identical bits."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = preset("c1").replace(precision="f32", N=N, n_particles=20000)
+    rng = np.random.default_rng(3)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, _ = ic.initial_droplets(cfg, rng)
+    outs = []
+    for late in ("0", "1"):
+        monkeypatch.setenv("LD_LATE_DENSITY", late)
+        d = ForcedDNS(cfg)
+        t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+        d.set_state(*t)
+        d.advance(dt, 3)
+        d.get_state(*t)
+        torch.cuda.synchronize()
+        outs.append([x.cpu().numpy() for x in t])
+        d.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_ensemble_matches_single_runs(gpu, prec):
     """A batched ensemble (one launch per kernel for all realizations) evolves
