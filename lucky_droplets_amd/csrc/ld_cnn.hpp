@@ -469,17 +469,13 @@ k_wgrad2(const bf16* __restrict__ XB, const bf16* __restrict__ dZB, float* __res
   float db[MT];
 #pragma unroll
   for (int m = 0; m < MT; ++m) db[m] = 0.f;
-  const size_t kb = (size_t)blockIdx.x * chunk;
-#pragma unroll 2
-  for (int ks = 0; ks < chunk / 32; ++ks) {
-    const size_t nq = kb + 32 * ks + 8 * q;
-    const int zq = (int)(nq & msk);
-    const size_t line = nq / N;
-    const int x = (int)(line / N), y = (int)(line & msk);
+  // one K-step: the lane's 8 nodes start at z = zq on line (x, y) (8-node
+  // group gq of the dZ copy); A from dZ, B from the neighbour lines' X copy
+  auto kstep = [&](size_t gq, int zq, int x, int y) {
     bf16x8 a[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
-      const uint4 v = *reinterpret_cast<const uint4*>(dZB + ((nq >> 3) * CO + m * 16 + n16) * 8);
+      const uint4 v = *reinterpret_cast<const uint4*>(dZB + (gq * CO + m * 16 + n16) * 8);
       a[m] = __builtin_bit_cast(bf16x8, v);
       if (w == 0) {
         const uint32_t u[4] = {v.x, v.y, v.z, v.w};
@@ -487,18 +483,21 @@ k_wgrad2(const bf16* __restrict__ XB, const bf16* __restrict__ dZB, float* __res
         for (int i = 0; i < 4; ++i) db[m] += b2f((bf16)(u[i] & 0xffff)) + b2f((bf16)(u[i] >> 16));
       }
     }
+    const int G = N >> 3;                          // 8-node groups per z-line
 #pragma unroll
     for (int li = 0; li < LPW; ++li) {
       const int wl = w * LPW + li;                 // neighbour line index 0..8
       const int dx = wl / 3 - 1, dy = wl % 3 - 1;
-      const size_t base = ((size_t)((x + dx) & msk) * N + ((y + dy) & msk)) * N;
+      const size_t gb = ((size_t)((x + dx) & msk) * N + ((y + dy) & msk)) * G;   // its first group
+      const size_t cg = gb + (zq >> 3);
+      const size_t pg = zq ? cg - 1 : gb + G - 1;                                // periodic in z
+      const size_t ng = zq + 8 < N ? cg + 1 : gb;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const int ci = t * 16 + n16;
-        const size_t np = base + ((zq - 2) & msk), nn = base + ((zq + 8) & msk);
-        const uint4 c = *reinterpret_cast<const uint4*>(XB + (((base + zq) >> 3) * CI + ci) * 8);
-        const uint32_t pv = *reinterpret_cast<const uint32_t*>(XB + ((np >> 3) * CI + ci) * 8 + (np & 7));
-        const uint32_t nv = *reinterpret_cast<const uint32_t*>(XB + ((nn >> 3) * CI + ci) * 8 + (nn & 7));
+        const uint4 c = *reinterpret_cast<const uint4*>(XB + (cg * CI + ci) * 8);
+        const uint32_t pv = *reinterpret_cast<const uint32_t*>(XB + (pg * CI + ci) * 8 + 6);
+        const uint32_t nv = *reinterpret_cast<const uint32_t*>(XB + (ng * CI + ci) * 8);
         const uint32_t s1 = __builtin_amdgcn_alignbyte(c.y, c.x, 2), s2 = __builtin_amdgcn_alignbyte(c.z, c.y, 2),
                        s3 = __builtin_amdgcn_alignbyte(c.w, c.z, 2);
         const bf16x8 fm = __builtin_bit_cast(bf16x8, make_uint4(__builtin_amdgcn_alignbyte(c.x, pv, 2), s1, s2, s3));
@@ -511,6 +510,29 @@ k_wgrad2(const bf16* __restrict__ XB, const bf16* __restrict__ dZB, float* __res
           acc[3 * li + 2][m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], fp, acc[3 * li + 2][m][t], 0, 0, 0);
         }
       }
+    }
+  };
+  const size_t kb = (size_t)blockIdx.x * chunk;
+  if (N >= 32) {
+    // the chunk is whole z-lines: line coordinates once per line, K-steps
+    // along z with plain increments
+    const int lines = chunk / N;
+    const size_t l0 = kb / N;
+    for (int li0 = 0; li0 < lines; ++li0) {
+      const size_t line = l0 + li0;
+      const int x = (int)(line / N), y = (int)(line & msk);
+      const size_t gl = line * (size_t)(N >> 3);
+#pragma unroll 2
+      for (int z0 = 0; z0 < N; z0 += 32) {
+        const int zq = z0 + 8 * q;
+        kstep(gl + (zq >> 3), zq, x, y);
+      }
+    }
+  } else {
+    for (int ks = 0; ks < chunk / 32; ++ks) {
+      const size_t nq = kb + 32 * ks + 8 * q;
+      const size_t line = nq / N;
+      kstep(nq >> 3, (int)(nq & msk), (int)(line / N), (int)(line & msk));
     }
   }
   float* out = part + (size_t)blockIdx.x * (27 * CO * CI + CO);
