@@ -81,6 +81,11 @@ struct Engine final : EngineBase {
   // queue a stage's density chain after the next flow x pass (measured at 256^3:
   // 8.18 vs 8.31 ms/step; the 64^3 ensemble is faster in the plain order)
   bool late_density = N >= 128;
+  // Dealiasing truncation: every mode with kz > N/3 is outside the mask
+  // (kint >= kz), so the nonlinear term there is discarded and v_hat is zero.
+  // The passes that carry only those two quantities skip the kz > N/3 lines
+  // (kc = N/3); kc = N/2 runs every line.
+  int kc = N / 3;
   hipEvent_t evA = nullptr, evB = nullptr;
   C<T>*uk = nullptr, *us = nullptr, *acc = nullptr, *w6 = nullptr, *wn = nullptr;
   T* ubuf[2] = {nullptr, nullptr};       // planar physical u (API / plain paths)
@@ -228,6 +233,7 @@ struct Engine final : EngineBase {
     // two chains never run concurrently (per-kernel timing without overlap)
     if (const char* e = getenv("LD_SERIAL")) serial = atoi(e) != 0;
     if (const char* e = getenv("LD_LATE_DENSITY")) late_density = atoi(e) != 0;
+    if (const char* e = getenv("LD_TRUNCATE")) kc = atoi(e) != 0 ? N / 3 : N / 2;
     if (const char* e = getenv("LD_PSTAGE_LDS")) {
       const int v = atoi(e);
       if (v >= 0 && v <= 65536) pstage_lds = (unsigned)v;
@@ -304,29 +310,29 @@ struct Engine final : EngineBase {
   static constexpr int MH = M / 2, M2 = 2 * M;
   int my = M;
   template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale, int x0 = 0,
-                                        int nx = N) {
+                                        int nx = N, int kcut = N / 2) {
     const int kid = AXIS == 1 ? LD_K_YPASS : -2;
     const int qoff = AXIS == 1 ? x0 * Nf : 0;
     LD_TRY(prof_mark(kid, s));
     if (AXIS == 1 && my == MH)
       hipLaunchKernelGGL((k_pass<T, N, MH, AXIS, DIR>), dim3(nx * Nf / MH, nf, B), dim3(MH * N1), 0, s, in, out, CF,
-                         (const C<T>*)tw, scale, qoff, rs);
+                         (const C<T>*)tw, scale, qoff, rs, kcut);
     else if (AXIS == 1 && my == M2)
       hipLaunchKernelGGL((k_pass<T, N, M2, AXIS, DIR>), dim3(nx * Nf / M2, nf, B), dim3(M2 * N1), 0, s, in, out, CF,
-                         (const C<T>*)tw, scale, qoff, rs);
+                         (const C<T>*)tw, scale, qoff, rs, kcut);
     else
       hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(AXIS == 1 ? nx * Nf / M : NBLK_X, nf, B), dim3(M * N1), 0, s,
-                         in, out, CF, (const C<T>*)tw, scale, qoff, rs);
+                         in, out, CF, (const C<T>*)tw, scale, qoff, rs, kcut);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
   }
   // z pass over the (x, y) lines of x-planes [x0, x0 + nx).
   template <int P, int FIN, int FOUT, class Op> int zrun(hipStream_t s, const Op& op, int kid = -2, int x0 = 0,
-                                                         int nx = N) {
+                                                         int nx = N, int kcin = N / 2, int kcout = N / 2) {
     LD_TRY(prof_mark(kid, s));
     hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(nx * N / P, 1, B), dim3(ZThreads<N, P, FIN, FOUT>::v), 0,
-                       s, op, (const C<T>*)tw, x0 * N, rs);
+                       s, op, (const C<T>*)tw, x0 * N, rs, kcin, kcout);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
@@ -458,8 +464,8 @@ struct Engine final : EngineBase {
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx)));
-      LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx)));
-      LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx)));
+      LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, N / 2, kc)));
+      LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx, kc)));
     }
     return LD_OK;
   }
@@ -471,9 +477,9 @@ struct Engine final : EngineBase {
     OpZVel<T, N, P3> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
-      LD_TRY((pass<1, +1>(s, w6, w6, 3, inv(), x0, nx)));
+      LD_TRY((pass<1, +1>(s, w6, w6, 3, inv(), x0, nx, kc)));
       if (x0 == 0 && wait) LD_HIP(hipStreamWaitEvent(s, wait, 0));
-      LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx)));
+      LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc, N / 2)));
     }
     return LD_OK;
   }
@@ -533,6 +539,7 @@ struct Engine final : EngineBase {
     ra.nl = w6; ra.us = usp; ra.ubase = uk; ra.acc = acc; ra.usnext = (s < 3) ? us : nullptr; ra.ku = nullptr;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
     ra.wacc = (T)wh; ra.anext = (T)anext; ra.scale = inv(); ra.acc_init = (s == 0);
+    ra.kcut = kc;
     LD_TRY(prof_mark(LD_K_XRHS, st));
     hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw, rs);
     LD_LAUNCHED();
@@ -836,6 +843,7 @@ struct Engine final : EngineBase {
     ra.nl = w6; ra.us = ukp; ra.ubase = ukp; ra.acc = nullptr; ra.usnext = nullptr; ra.ku = (C<T>*)ku;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit * visc); ra.tps = (T)p.tps;
     ra.wacc = (T)0; ra.anext = (T)0; ra.scale = inv(); ra.acc_init = 0;
+    ra.kcut = kc;
     hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw, rs);
     LD_LAUNCHED();
     // the physical-space v the droplets see uses visc as well (forced-dns-sm.py:319-321)

@@ -109,9 +109,12 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
 // One line pass along x or y for `gridDim.y` fields (field stride fstride).
 // DIR < 0: forward (input natural order, output natural order);
 // DIR > 0: inverse times `scale`.  In place is allowed (each block owns its lines).
+// y passes: lines with kz > kcut are neither read nor written (their modes
+// are zero on input or masked by every consumer of the output).
 template <typename T, int N, int M, int AXIS, int DIR>
 __global__ void __launch_bounds__(M * Plan<N>::N1)
-k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff, size_t rs) {
+k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff, size_t rs,
+       int kcut) {
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, js = JS<N, AXIS>::v;
   __shared__ C<T> lds[M * (N + 1)];
   const int m = threadIdx.x % M, n1 = threadIdx.x / M;
@@ -124,19 +127,34 @@ k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T
   C<T>* dst = rsel1(out, rs) + (size_t)blockIdx.y * fstride;
   const uint32_t lb = (uint32_t)lineBase<N, AXIS>(q);
   C<T>* L = lds + m * (N + 1);
+  const bool live = AXIS != 1 || q % (N / 2 + 1) <= kcut;
   C<T> a[N2];
   if constexpr (DIR < 0) {
+    if (live) {
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(src, lb + (uint32_t)(posIn<N>(n1, n2) * js));
+      for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(src, lb + (uint32_t)(posIn<N>(n1, n2) * js));
+    } else {
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) a[n2] = mkc<T>((T)0, (T)0);
+    }
     lineFwd<T, N>(a, L, n1, true, tw);
+    if (live) {
 #pragma unroll
-    for (int r = 0; r < N2; ++r) st32(dst, lb + (uint32_t)(posOut<N>(n1, r) * js), a[r]);
+      for (int r = 0; r < N2; ++r) st32(dst, lb + (uint32_t)(posOut<N>(n1, r) * js), a[r]);
+    }
   } else {
+    if (live) {
 #pragma unroll
-    for (int r = 0; r < N2; ++r) a[r] = ld32(src, lb + (uint32_t)(posOut<N>(n1, r) * js));
+      for (int r = 0; r < N2; ++r) a[r] = ld32(src, lb + (uint32_t)(posOut<N>(n1, r) * js));
+    } else {
+#pragma unroll
+      for (int r = 0; r < N2; ++r) a[r] = mkc<T>((T)0, (T)0);
+    }
     lineInv<T, N>(a, L, n1, true, tw);
+    if (live) {
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) st32(dst, lb + (uint32_t)(posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
+      for (int n2 = 0; n2 < N2; ++n2) st32(dst, lb + (uint32_t)(posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
+    }
   }
 }
 
@@ -258,10 +276,12 @@ template <typename T> struct alignas(2 * sizeof(C<T>)) C2 { C<T> a, b; };
 // P*Nf entries per field, kept in LDS in the same order (the regions of
 // consecutive line pairs are adjacent), so they move as pairs of entries;
 // line0 * Nf is even (P and the line offset are even), so every pair is
-// aligned.
+// aligned.  Spectral entries with kz > kcin are taken as zero without being
+// read; entries with kz > kcout are not written (pairs holding one entry on
+// each side are moved whole).
 template <typename T, int N, int P, int FIN, int FOUT, class Op>
 __global__ void __launch_bounds__((ZThreads<N, P, FIN, FOUT>::v))
-k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
+k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcin, int kcout) {
   static_assert(P % 2 == 0, "pairs of lines");
   const Op op = op0.shifted(rs);
   constexpr int Nf = N / 2 + 1;
@@ -276,7 +296,15 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
   if constexpr (FIN > 0) {
     for (int e = threadIdx.x; e < FIN * (NE / 2); e += nt) {
       const int f = e / (NE / 2), v = e % (NE / 2);
-      lds2[f * (NE / 2) + v] = reinterpret_cast<const C2<T>*>(op.specIn(f) + g0)[v];
+      const int za = (2 * v) % Nf, zb = (2 * v + 1) % Nf;
+      C2<T> x;
+      x.a = mkc<T>((T)0, (T)0); x.b = x.a;
+      if (za <= kcin || zb <= kcin) {
+        x = reinterpret_cast<const C2<T>*>(op.specIn(f) + g0)[v];
+        if (za > kcin) x.a = mkc<T>((T)0, (T)0);
+        if (zb > kcin) x.b = mkc<T>((T)0, (T)0);
+      }
+      lds2[f * (NE / 2) + v] = x;
     }
     __syncthreads();
     zInverse<T, N, P, FIN>(tile, tw);
@@ -287,6 +315,7 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
     zForward<T, N, P, FOUT>(tile, tw);
     for (int e = threadIdx.x; e < FOUT * (NE / 2); e += nt) {
       const int f = e / (NE / 2), q = 2 * (e % (NE / 2));
+      if (q % Nf > kcout && (q + 1) % Nf > kcout) continue;
       C2<T> v;
       v.a = zSplit<T, N, P>(tile, f, q / Nf, q % Nf);
       v.b = zSplit<T, N, P>(tile, f, (q + 1) / Nf, (q + 1) % Nf);
@@ -529,6 +558,8 @@ template <typename T> struct RhsArgs {
   size_t fs;
   T nu_eff, tps, wacc, anext, scale;
   int acc_init;
+  int kcut;            // lines (ky, kz) with kz > kcut: the nonlinear term is masked there and v_hat is
+                       // zero, so neither is read or written
   __device__ RhsArgs shifted(size_t rs) const {
     RhsArgs a = *this;
     a.nl = rsel1(nl, rs); a.us = rsel1(us, rs); a.ubase = rsel1(ubase, rs); a.acc = rsel(acc, rs);
@@ -620,9 +651,15 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
   C<T>* Lf = lds + (f * M + m) * (N + 1);
   const size_t fs = A.fs;
   C<T>* NLf = A.nl + f * fs;
+  const bool live = iz <= A.kcut;
   C<T> a[N2];
+  if (live) {
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js));
+    for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js));
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) a[n2] = mkc<T>((T)0, (T)0);
+  }
   lineFwd<T, N>(a, Lf, n1, true, tw);
   __syncthreads();
 #pragma unroll
@@ -682,8 +719,10 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
 #pragma unroll
   for (int r = 0; r < N2; ++r) a[r] = Lf[posOut<N>(n1, r)];
   lineInv<T, N>(a, Lf, n1, true, tw);
+  if (live) {
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
+    for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
+  }
 }
 
 // Step epilogue in spectral space (forced-dns-sm.py:597-610, 618): the

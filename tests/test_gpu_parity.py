@@ -417,6 +417,36 @@ def test_density_chain_order_keeps_results(gpu, N, monkeypatch):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+@pytest.mark.parametrize("N,prec", [(64, "f64"), (64, "f32"), (256, "f32")])
+def test_dealias_truncation_keeps_results(gpu, N, prec, monkeypatch):
+    """The passes that skip the kz > N/3 lines (LD_TRUNCATE=1, the default)
+    give the same values as running every line (0): those lines carry only
+    modes that the dealiasing mask removes (the nonlinear term) or that are
+    zero (v_hat).  Compared as values: a skipped zero may differ in sign."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = preset("c1").replace(precision=prec, N=N, n_particles=20000)
+    rng = np.random.default_rng(5)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, _ = ic.initial_droplets(cfg, rng)
+    outs = []
+    for trunc in ("0", "1"):
+        monkeypatch.setenv("LD_TRUNCATE", trunc)
+        d = ForcedDNS(cfg)
+        t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+        d.set_state(*t)
+        d.advance(dt, 3)
+        d.get_state(*t)
+        torch.cuda.synchronize()
+        outs.append([x.cpu().numpy() for x in t])
+        d.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_ensemble_matches_single_runs(gpu, prec):
     """A batched ensemble (one launch per kernel for all realizations) evolves
