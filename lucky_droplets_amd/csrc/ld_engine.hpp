@@ -83,9 +83,15 @@ struct Engine final : EngineBase {
   bool late_density = N >= 128;
   // Dealiasing truncation: every mode with kz > N/3 is outside the mask
   // (kint >= kz), so the nonlinear term there is discarded and v_hat is zero.
-  // The passes that carry only those two quantities skip the kz > N/3 lines
-  // (kc = N/3); kc = N/2 runs every line.
+  // The in-place y passes skip the kz > N/3 lines (kc = N/3; kc = N/2 runs
+  // every line) where they carry the nonlinear term (x_rhs3 does not read
+  // those lines) or v_hat (zero lines, already zero after the x pass).
   int kc = N / 3;
+  // u_hat of the context state has no nonzero mode with kz > kc (checked when
+  // the state is set; the stepping keeps it): the y pass of u, omega skips
+  // those lines as well (zero after the x pass)
+  bool band = false;
+  int* bandflag = nullptr;
   hipEvent_t evA = nullptr, evB = nullptr;
   C<T>*uk = nullptr, *us = nullptr, *acc = nullptr, *w6 = nullptr, *wn = nullptr;
   T* ubuf[2] = {nullptr, nullptr};       // planar physical u (API / plain paths)
@@ -279,6 +285,7 @@ struct Engine final : EngineBase {
     }
     LD_TRY(arena_commit());
     LD_TRY(alloc(shells, 8));
+    LD_TRY(alloc(bandflag, 1));
     LD_HIP(hipMemcpy(shells, p.shell_no, 8 * sizeof(int), hipMemcpyHostToDevice));
     const double cn[4] = {1.0, 1.0, 0.0, 0.0};
     for (int b = 0; b < B; ++b) {
@@ -329,10 +336,10 @@ struct Engine final : EngineBase {
   }
   // z pass over the (x, y) lines of x-planes [x0, x0 + nx).
   template <int P, int FIN, int FOUT, class Op> int zrun(hipStream_t s, const Op& op, int kid = -2, int x0 = 0,
-                                                         int nx = N, int kcin = N / 2, int kcout = N / 2) {
+                                                         int nx = N) {
     LD_TRY(prof_mark(kid, s));
     hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(nx * N / P, 1, B), dim3(ZThreads<N, P, FIN, FOUT>::v), 0,
-                       s, op, (const C<T>*)tw, x0 * N, rs, kcin, kcout);
+                       s, op, (const C<T>*)tw, x0 * N, rs);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
@@ -459,12 +466,13 @@ struct Engine final : EngineBase {
     LD_TRY(prof_mark(LD_K_XUW, s));
     return LD_OK;
   }
-  int nonlinear_yz(hipStream_t s, T* uphys, V4<T>* g) {
+  // kcu: lines with kz > kcu of the stage u_hat are known to be zero
+  int nonlinear_yz(hipStream_t s, T* uphys, V4<T>* g, int kcu = N / 2) {
     OpZCross<T, N, P6> oc{w6, CF, uphys, g, inv()};
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
-      LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx)));
-      LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, N / 2, kc)));
+      LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx, kcu)));
+      LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx)));
       LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx, kc)));
     }
     return LD_OK;
@@ -479,7 +487,7 @@ struct Engine final : EngineBase {
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, w6, w6, 3, inv(), x0, nx, kc)));
       if (x0 == 0 && wait) LD_HIP(hipStreamWaitEvent(s, wait, 0));
-      LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc, N / 2)));
+      LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx)));
     }
     return LD_OK;
   }
@@ -534,7 +542,7 @@ struct Engine final : EngineBase {
     V4<T>* gup = gu[s & 1];
     const double wh = (s == 0 || s == 3) ? h / 6.0 : h / 3.0;   // forced-dns-sm.py:342-358
     const double anext = a_coef[s + 1] * h;
-    LD_TRY(nonlinear_yz(st, nullptr, gup));
+    LD_TRY(nonlinear_yz(st, nullptr, gup, band ? kc : N / 2));
     RhsArgs<T> ra;
     ra.nl = w6; ra.us = usp; ra.ubase = uk; ra.acc = acc; ra.usnext = (s < 3) ? us : nullptr; ra.ku = nullptr;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
@@ -672,7 +680,26 @@ struct Engine final : EngineBase {
                             hipMemcpyDeviceToDevice, st));
       LD_HIP(hipMemcpyAsync(at(n_state, b), (const T*)nin + (size_t)b * RF, RF * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
+    LD_TRY(check_band());
     return set_rows(coord, n_);
+  }
+  // band <- no realization's u_hat has a nonzero mode with kz > kc (one
+  // synchronisation, when the state is set)
+  int check_band() {
+    band = false;
+    if (kc >= N / 2) return LD_OK;
+    hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(256), 0, st, bandflag, (size_t)1, 0, (size_t)0);
+    LD_LAUNCHED();
+    for (int b = 0; b < B; ++b) {
+      hipLaunchKernelGGL((k_band_check<T, N>), dim3(1024), dim3(256), 0, st, (const C<T>*)at(uk, b),
+                         (size_t)3 * N * N, kc, bandflag);
+      LD_LAUNCHED();
+    }
+    int h = 1;
+    LD_HIP(hipMemcpyAsync(&h, bandflag, sizeof(int), hipMemcpyDeviceToHost, st));
+    LD_HIP(hipStreamSynchronize(st));
+    band = h == 0;
+    return LD_OK;
   }
   int state_get(void* ukout, void* nout, double* coord) override {
     LD_TRY(join(st, st2, evB));
@@ -726,7 +753,7 @@ struct Engine final : EngineBase {
   // hipGraphLaunch.  The capture runs the host side of the step (the sort's
   // buffer swap included) without executing it; a replay repeats that swap.
   // Steps run on an internal stream, joined to the caller's stream by events.
-  struct StepGraph { hipGraphExec_t exec = nullptr; double h = 0.0; int64_t np = -1; };
+  struct StepGraph { hipGraphExec_t exec = nullptr; double h = 0.0; int64_t np = -1; bool band = false; };
   bool use_graph = false;
   hipStream_t stg = nullptr;
   hipEvent_t evG0 = nullptr, evG1 = nullptr;   // caller <-> internal stream joins
@@ -740,7 +767,7 @@ struct Engine final : EngineBase {
   }
   int graph_step(double h) {
     StepGraph& g = graphs[parity];
-    if (!g.exec || g.h != h || g.np != np) {
+    if (!g.exec || g.h != h || g.np != np || g.band != band) {
       if (g.exec) { LD_HIP(hipGraphExecDestroy(g.exec)); g.exec = nullptr; }
       const int par = parity;
       LD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -759,6 +786,7 @@ struct Engine final : EngineBase {
       LD_HIP(ei);
       graphs[par].h = h;
       graphs[par].np = np;
+      graphs[par].band = band;
       LD_HIP(hipGraphLaunch(graphs[par].exec, st));
       return LD_OK;
     }
@@ -1003,6 +1031,7 @@ struct Engine final : EngineBase {
       LD_LAUNCHED();
     }
     if (np > 0) LD_TRY(set_rows(rows_c, np));
+    LD_TRY(check_band());   // the transforms' round-off reaches every mode
     int32_t blown = 0;
     LD_HIP(hipMemcpyAsync(&blown, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     LD_HIP(hipStreamSynchronize(st));

@@ -109,8 +109,10 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
 // One line pass along x or y for `gridDim.y` fields (field stride fstride).
 // DIR < 0: forward (input natural order, output natural order);
 // DIR > 0: inverse times `scale`.  In place is allowed (each block owns its lines).
-// y passes: lines with kz > kcut are neither read nor written (their modes
-// are zero on input or masked by every consumer of the output).
+// y passes: lines with kz > kcut are neither read nor written.  Callers pass
+// kcut < N/2 only in place (in == out) and only where those lines are zero
+// (their transform is zero, which they already hold) or are never read
+// afterwards (modes the dealiasing mask removes).
 template <typename T, int N, int M, int AXIS, int DIR>
 __global__ void __launch_bounds__(M * Plan<N>::N1)
 k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff, size_t rs,
@@ -276,12 +278,10 @@ template <typename T> struct alignas(2 * sizeof(C<T>)) C2 { C<T> a, b; };
 // P*Nf entries per field, kept in LDS in the same order (the regions of
 // consecutive line pairs are adjacent), so they move as pairs of entries;
 // line0 * Nf is even (P and the line offset are even), so every pair is
-// aligned.  Spectral entries with kz > kcin are taken as zero without being
-// read; entries with kz > kcout are not written (pairs holding one entry on
-// each side are moved whole).
+// aligned.
 template <typename T, int N, int P, int FIN, int FOUT, class Op>
 __global__ void __launch_bounds__((ZThreads<N, P, FIN, FOUT>::v))
-k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcin, int kcout) {
+k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
   static_assert(P % 2 == 0, "pairs of lines");
   const Op op = op0.shifted(rs);
   constexpr int Nf = N / 2 + 1;
@@ -296,15 +296,7 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcin, int k
   if constexpr (FIN > 0) {
     for (int e = threadIdx.x; e < FIN * (NE / 2); e += nt) {
       const int f = e / (NE / 2), v = e % (NE / 2);
-      const int za = (2 * v) % Nf, zb = (2 * v + 1) % Nf;
-      C2<T> x;
-      x.a = mkc<T>((T)0, (T)0); x.b = x.a;
-      if (za <= kcin || zb <= kcin) {
-        x = reinterpret_cast<const C2<T>*>(op.specIn(f) + g0)[v];
-        if (za > kcin) x.a = mkc<T>((T)0, (T)0);
-        if (zb > kcin) x.b = mkc<T>((T)0, (T)0);
-      }
-      lds2[f * (NE / 2) + v] = x;
+      lds2[f * (NE / 2) + v] = reinterpret_cast<const C2<T>*>(op.specIn(f) + g0)[v];
     }
     __syncthreads();
     zInverse<T, N, P, FIN>(tile, tw);
@@ -315,7 +307,6 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcin, int k
     zForward<T, N, P, FOUT>(tile, tw);
     for (int e = threadIdx.x; e < FOUT * (NE / 2); e += nt) {
       const int f = e / (NE / 2), q = 2 * (e % (NE / 2));
-      if (q % Nf > kcout && (q + 1) % Nf > kcout) continue;
       C2<T> v;
       v.a = zSplit<T, N, P>(tile, f, q / Nf, q % Nf);
       v.b = zSplit<T, N, P>(tile, f, (q + 1) / Nf, (q + 1) % Nf);
@@ -558,8 +549,8 @@ template <typename T> struct RhsArgs {
   size_t fs;
   T nu_eff, tps, wacc, anext, scale;
   int acc_init;
-  int kcut;            // lines (ky, kz) with kz > kcut: the nonlinear term is masked there and v_hat is
-                       // zero, so neither is read or written
+  int kcut;            // lines (ky, kz) with kz > kcut: the nonlinear term is masked there, so it is
+                       // not read (those lines of nl may hold data of an earlier pass)
   __device__ RhsArgs shifted(size_t rs) const {
     RhsArgs a = *this;
     a.nl = rsel1(nl, rs); a.us = rsel1(us, rs); a.ubase = rsel1(ubase, rs); a.acc = rsel(acc, rs);
@@ -626,6 +617,24 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
   C<T>* Ou = out + f * fs;
 #pragma unroll
   for (int n2 = 0; n2 < N2; ++n2) st32(Ou, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(u[n2], scale));
+}
+
+// flag[0] <- 1 if any of the `lines` spectral lines of u (Nf entries each) has
+// a nonzero (or NaN) entry with kz > kc; flag[0] is cleared beforehand.  Those
+// modes are outside the dealiasing mask, and a state without them keeps none
+// through the RK4 stages and the step epilogue.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_band_check(const C<T>* __restrict__ u, size_t lines, int kc, int* flag) {
+  constexpr int Nf = N / 2 + 1;
+  const int w = Nf - 1 - kc;
+  const size_t tot = lines * (size_t)w;
+  bool any = false;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (size_t)gridDim.x * 256) {
+    const size_t line = e / (size_t)w;
+    const C<T> v = u[line * Nf + (size_t)(kc + 1) + e % (size_t)w];
+    any = any || !(v.x == (T)0 && v.y == (T)0);
+  }
+  if (__syncthreads_or(any) && threadIdx.x == 0) flag[0] = 1;
 }
 
 // Nonlinear term + forcing + dealiasing + pressure projection + viscosity
@@ -719,10 +728,8 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
 #pragma unroll
   for (int r = 0; r < N2; ++r) a[r] = Lf[posOut<N>(n1, r)];
   lineInv<T, N>(a, Lf, n1, true, tw);
-  if (live) {
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
-  }
+  for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
 }
 
 // Step epilogue in spectral space (forced-dns-sm.py:597-610, 618): the

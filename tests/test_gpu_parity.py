@@ -417,12 +417,16 @@ def test_density_chain_order_keeps_results(gpu, N, monkeypatch):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
-@pytest.mark.parametrize("N,prec", [(64, "f64"), (64, "f32"), (256, "f32")])
-def test_dealias_truncation_keeps_results(gpu, N, prec, monkeypatch):
+@pytest.mark.parametrize("N,prec,outside", [(64, "f64", False), (64, "f32", False), (256, "f32", False),
+                                            (64, "f64", True), (64, "f32", True)])
+def test_dealias_truncation_keeps_results(gpu, N, prec, outside, monkeypatch):
     """The passes that skip the kz > N/3 lines (LD_TRUNCATE=1, the default)
     give the same values as running every line (0): those lines carry only
     modes that the dealiasing mask removes (the nonlinear term) or that are
-    zero (v_hat).  Compared as values: a skipped zero may differ in sign."""
+    zero (v_hat, and u_hat when the state has none there).  With `outside`
+    the state carries small modes at kz > N/3, which the state check must see
+    (the u, omega transforms then run every line).  Compared as values: a
+    skipped zero may differ in sign."""
     import torch
     from lucky_droplets_amd import ic
     from lucky_droplets_amd.config import preset
@@ -432,10 +436,20 @@ def test_dealias_truncation_keeps_results(gpu, N, prec, monkeypatch):
     uk, n, u = ic.initial_flow(cfg, rng)
     dt = ic.cfl_dt(cfg, u)
     coord, _ = ic.initial_droplets(cfg, rng)
+    uk0 = uk
+    if outside:
+        kc = N // 3
+        sh = uk[:, :, :, kc + 1:].shape
+        uk = uk.copy()
+        uk[:, :, :, kc + 1:] = 1e-3 * np.abs(uk).max() * (rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
     outs = []
     for trunc in ("0", "1"):
         monkeypatch.setenv("LD_TRUNCATE", trunc)
-        d = ForcedDNS(cfg)
+        d = ForcedDNS(cfg, graph=True)
+        if outside and trunc == "1":
+            # a step graph captured for a band-limited state must not be replayed for this one
+            d.set_state(_t(uk0, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+            d.advance(dt, 1)
         t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
         d.set_state(*t)
         d.advance(dt, 3)
