@@ -319,17 +319,20 @@ struct Engine final : EngineBase {
   template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale, int x0 = 0,
                                         int nx = N, int kcut = N / 2) {
     const int kid = AXIS == 1 ? LD_K_YPASS : -2;
-    const int qoff = AXIS == 1 ? x0 * Nf : 0;
+    // y passes: only the lines with kz <= kcut, lw of them per x-plane
+    const int lw = (kcut < N / 2 ? kcut : N / 2) + 1;
+    const int qoff = AXIS == 1 ? x0 * lw : 0, qend = AXIS == 1 ? (x0 + nx) * lw : 0;
+    const int nl = nx * lw;
     LD_TRY(prof_mark(kid, s));
     if (AXIS == 1 && my == MH)
-      hipLaunchKernelGGL((k_pass<T, N, MH, AXIS, DIR>), dim3(nx * Nf / MH, nf, B), dim3(MH * N1), 0, s, in, out, CF,
-                         (const C<T>*)tw, scale, qoff, rs, kcut);
+      hipLaunchKernelGGL((k_pass<T, N, MH, AXIS, DIR>), dim3((nl + MH - 1) / MH, nf, B), dim3(MH * N1), 0, s, in, out,
+                         CF, (const C<T>*)tw, scale, qoff, rs, lw, qend);
     else if (AXIS == 1 && my == M2)
-      hipLaunchKernelGGL((k_pass<T, N, M2, AXIS, DIR>), dim3(nx * Nf / M2, nf, B), dim3(M2 * N1), 0, s, in, out, CF,
-                         (const C<T>*)tw, scale, qoff, rs, kcut);
+      hipLaunchKernelGGL((k_pass<T, N, M2, AXIS, DIR>), dim3((nl + M2 - 1) / M2, nf, B), dim3(M2 * N1), 0, s, in, out,
+                         CF, (const C<T>*)tw, scale, qoff, rs, lw, qend);
     else
-      hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(AXIS == 1 ? nx * Nf / M : NBLK_X, nf, B), dim3(M * N1), 0, s,
-                         in, out, CF, (const C<T>*)tw, scale, qoff, rs, kcut);
+      hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(AXIS == 1 ? (nl + M - 1) / M : NBLK_X, nf, B), dim3(M * N1),
+                         0, s, in, out, CF, (const C<T>*)tw, scale, qoff, rs, lw, qend);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;

@@ -109,14 +109,16 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
 // One line pass along x or y for `gridDim.y` fields (field stride fstride).
 // DIR < 0: forward (input natural order, output natural order);
 // DIR > 0: inverse times `scale`.  In place is allowed (each block owns its lines).
-// y passes: lines with kz > kcut are neither read nor written.  Callers pass
-// kcut < N/2 only in place (in == out) and only where those lines are zero
-// (their transform is zero, which they already hold) or are never read
-// afterwards (modes the dealiasing mask removes).
+// y passes run over the lines (x, kz) with kz < lw, numbered x * lw + kz,
+// from qoff to qend (lw = N/2 + 1: every line).  Lines with kz >= lw are
+// neither read nor written: callers pass lw < N/2 + 1 only in place (in ==
+// out) and only where those lines are zero (their transform is zero, which
+// they already hold) or are never read afterwards (modes the dealiasing mask
+// removes).
 template <typename T, int N, int M, int AXIS, int DIR>
 __global__ void __launch_bounds__(M * Plan<N>::N1)
 k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff, size_t rs,
-       int kcut) {
+       int lw, int qend) {
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, js = JS<N, AXIS>::v;
   __shared__ C<T> lds[M * (N + 1)];
   const int m = threadIdx.x % M, n1 = threadIdx.x / M;
@@ -124,12 +126,17 @@ k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T
   // straddle cache lines (y rows start every Nf elements) meet in its L2
   const int nbx = gridDim.x;
   const int bx = (nbx % 8 == 0) ? (blockIdx.x % 8) * (nbx / 8) + blockIdx.x / 8 : blockIdx.x;
-  const int q = qoff + bx * M + m;
+  const int qc = qoff + bx * M + m;
+  int q = qc;
+  bool live = true;
+  if constexpr (AXIS == 1) {
+    if (lw != N / 2 + 1) q = (qc / lw) * (N / 2 + 1) + qc % lw;
+    live = qc < qend;
+  }
   const C<T>* src = rsel1(in, rs) + (size_t)blockIdx.y * fstride;
   C<T>* dst = rsel1(out, rs) + (size_t)blockIdx.y * fstride;
   const uint32_t lb = (uint32_t)lineBase<N, AXIS>(q);
   C<T>* L = lds + m * (N + 1);
-  const bool live = AXIS != 1 || q % (N / 2 + 1) <= kcut;
   C<T> a[N2];
   if constexpr (DIR < 0) {
     if (live) {
