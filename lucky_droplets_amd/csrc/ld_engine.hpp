@@ -87,6 +87,9 @@ struct Engine final : EngineBase {
   // every line) where they carry the nonlinear term (x_rhs3 does not read
   // those lines) or v_hat (zero lines, already zero after the x pass).
   int kc = N / 3;
+  // largest ky^2 + kz^2 of a (ky, kz) line holding a mode inside the mask
+  // (round|k| <= kc <=> |k|^2 <= kc (kc + 1)); -1: no per-entry skipping
+  int sphere() const { return kc < N / 2 ? kc * (kc + 1) : -1; }
   // u_hat of the context state has no nonzero mode with kz > kc (checked when
   // the state is set; the stepping keeps it): the y pass of u, omega skips
   // those lines as well (zero after the x pass)
@@ -317,7 +320,7 @@ struct Engine final : EngineBase {
   static constexpr int MH = M / 2, M2 = 2 * M;
   int my = M;
   template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale, int x0 = 0,
-                                        int nx = N, int kcut = N / 2) {
+                                        int nx = N, int kcut = N / 2, int r2 = -1) {
     const int kid = AXIS == 1 ? LD_K_YPASS : -2;
     // y passes: only the lines with kz <= kcut, lw of them per x-plane
     const int lw = (kcut < N / 2 ? kcut : N / 2) + 1;
@@ -326,13 +329,13 @@ struct Engine final : EngineBase {
     LD_TRY(prof_mark(kid, s));
     if (AXIS == 1 && my == MH)
       hipLaunchKernelGGL((k_pass<T, N, MH, AXIS, DIR>), dim3((nl + MH - 1) / MH, nf, B), dim3(MH * N1), 0, s, in, out,
-                         CF, (const C<T>*)tw, scale, qoff, rs, lw, qend);
+                         CF, (const C<T>*)tw, scale, qoff, rs, lw, qend, r2);
     else if (AXIS == 1 && my == M2)
       hipLaunchKernelGGL((k_pass<T, N, M2, AXIS, DIR>), dim3((nl + M2 - 1) / M2, nf, B), dim3(M2 * N1), 0, s, in, out,
-                         CF, (const C<T>*)tw, scale, qoff, rs, lw, qend);
+                         CF, (const C<T>*)tw, scale, qoff, rs, lw, qend, r2);
     else
       hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR>), dim3(AXIS == 1 ? (nl + M - 1) / M : NBLK_X, nf, B), dim3(M * N1),
-                         0, s, in, out, CF, (const C<T>*)tw, scale, qoff, rs, lw, qend);
+                         0, s, in, out, CF, (const C<T>*)tw, scale, qoff, rs, lw, qend, r2);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
@@ -476,7 +479,7 @@ struct Engine final : EngineBase {
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx, kcu)));
       LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx)));
-      LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx, kc)));
+      LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx, kc, sphere())));
     }
     return LD_OK;
   }
@@ -488,7 +491,7 @@ struct Engine final : EngineBase {
     OpZVel<T, N, P3> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
-      LD_TRY((pass<1, +1>(s, w6, w6, 3, inv(), x0, nx, kc)));
+      LD_TRY((pass<1, +1>(s, w6, w6, 3, inv(), x0, nx, kc, sphere())));
       if (x0 == 0 && wait) LD_HIP(hipStreamWaitEvent(s, wait, 0));
       LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx)));
     }

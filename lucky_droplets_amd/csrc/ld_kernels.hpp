@@ -114,11 +114,13 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
 // neither read nor written: callers pass lw < N/2 + 1 only in place (in ==
 // out) and only where those lines are zero (their transform is zero, which
 // they already hold) or are never read afterwards (modes the dealiasing mask
-// removes).
+// removes).  With r2 >= 0, only the spectral entries with ky^2 + kz^2 <= r2
+// are moved: a forward pass writes only those (the next pass reads no other),
+// an inverse pass reads only those (the others are zero).
 template <typename T, int N, int M, int AXIS, int DIR>
 __global__ void __launch_bounds__(M * Plan<N>::N1)
 k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff, size_t rs,
-       int lw, int qend) {
+       int lw, int qend, int r2) {
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, js = JS<N, AXIS>::v;
   __shared__ C<T> lds[M * (N + 1)];
   const int m = threadIdx.x % M, n1 = threadIdx.x / M;
@@ -127,12 +129,19 @@ k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T
   const int nbx = gridDim.x;
   const int bx = (nbx % 8 == 0) ? (blockIdx.x % 8) * (nbx / 8) + blockIdx.x / 8 : blockIdx.x;
   const int qc = qoff + bx * M + m;
-  int q = qc;
+  int q = qc, kz2 = 0;
   bool live = true;
   if constexpr (AXIS == 1) {
-    if (lw != N / 2 + 1) q = (qc / lw) * (N / 2 + 1) + qc % lw;
+    const int kz = qc % lw;
+    if (lw != N / 2 + 1) q = (qc / lw) * (N / 2 + 1) + kz;
+    kz2 = kz * kz;
     live = qc < qend;
   }
+  // spectral entry at y index iy is moved
+  auto keep = [&](int iy) {
+    const int ky = kwave(iy, N);
+    return AXIS != 1 || r2 < 0 || ky * ky + kz2 <= r2;
+  };
   const C<T>* src = rsel1(in, rs) + (size_t)blockIdx.y * fstride;
   C<T>* dst = rsel1(out, rs) + (size_t)blockIdx.y * fstride;
   const uint32_t lb = (uint32_t)lineBase<N, AXIS>(q);
@@ -149,12 +158,14 @@ k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T
     lineFwd<T, N>(a, L, n1, true, tw);
     if (live) {
 #pragma unroll
-      for (int r = 0; r < N2; ++r) st32(dst, lb + (uint32_t)(posOut<N>(n1, r) * js), a[r]);
+      for (int r = 0; r < N2; ++r)
+        if (keep(posOut<N>(n1, r))) st32(dst, lb + (uint32_t)(posOut<N>(n1, r) * js), a[r]);
     }
   } else {
     if (live) {
 #pragma unroll
-      for (int r = 0; r < N2; ++r) a[r] = ld32(src, lb + (uint32_t)(posOut<N>(n1, r) * js));
+      for (int r = 0; r < N2; ++r)
+        a[r] = keep(posOut<N>(n1, r)) ? ld32(src, lb + (uint32_t)(posOut<N>(n1, r) * js)) : mkc<T>((T)0, (T)0);
     } else {
 #pragma unroll
       for (int r = 0; r < N2; ++r) a[r] = mkc<T>((T)0, (T)0);
@@ -556,8 +567,9 @@ template <typename T> struct RhsArgs {
   size_t fs;
   T nu_eff, tps, wacc, anext, scale;
   int acc_init;
-  int kcut;            // lines (ky, kz) with kz > kcut: the nonlinear term is masked there, so it is
-                       // not read (those lines of nl may hold data of an earlier pass)
+  int kcut;            // kcut < N/2: lines (ky, kz) with ky^2 + kz^2 > kcut (kcut + 1) hold no mode
+                       // inside the mask, so their nonlinear term is not read (those entries of nl
+                       // may hold data of an earlier pass)
   __device__ RhsArgs shifted(size_t rs) const {
     RhsArgs a = *this;
     a.nl = rsel1(nl, rs); a.us = rsel1(us, rs); a.ubase = rsel1(ubase, rs); a.acc = rsel(acc, rs);
@@ -667,7 +679,8 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
   C<T>* Lf = lds + (f * M + m) * (N + 1);
   const size_t fs = A.fs;
   C<T>* NLf = A.nl + f * fs;
-  const bool live = iz <= A.kcut;
+  const int kyl = kwave(iy, N);
+  const bool live = A.kcut >= N / 2 || kyl * kyl + iz * iz <= A.kcut * (A.kcut + 1);
   C<T> a[N2];
   if (live) {
 #pragma unroll
