@@ -90,9 +90,10 @@ struct Engine final : EngineBase {
   // largest ky^2 + kz^2 of a (ky, kz) line holding a mode inside the mask
   // (round|k| <= kc <=> |k|^2 <= kc (kc + 1)); -1: no per-entry skipping
   int sphere() const { return kc < N / 2 ? kc * (kc + 1) : -1; }
-  // u_hat of the context state has no nonzero mode with kz > kc (checked when
-  // the state is set; the stepping keeps it): the y pass of u, omega skips
-  // those lines as well (zero after the x pass)
+  // u_hat of the context state has no nonzero mode outside the mask (checked
+  // when the state is set; the stepping keeps it): the y pass of u, omega
+  // skips the kz > kc lines and reads only the entries inside the mask radius
+  // as well (the others are zero after the x pass)
   bool band = false;
   int* bandflag = nullptr;
   hipEvent_t evA = nullptr, evB = nullptr;
@@ -472,12 +473,12 @@ struct Engine final : EngineBase {
     LD_TRY(prof_mark(LD_K_XUW, s));
     return LD_OK;
   }
-  // kcu: lines with kz > kcu of the stage u_hat are known to be zero
+  // kcu < N/2: the stage u_hat has no nonzero mode outside the mask
   int nonlinear_yz(hipStream_t s, T* uphys, V4<T>* g, int kcu = N / 2) {
     OpZCross<T, N, P6> oc{w6, CF, uphys, g, inv()};
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
-      LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx, kcu)));
+      LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx, kcu, kcu < N / 2 ? sphere() : -1)));
       LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx)));
       LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx, kc, sphere())));
     }
@@ -689,7 +690,7 @@ struct Engine final : EngineBase {
     LD_TRY(check_band());
     return set_rows(coord, n_);
   }
-  // band <- no realization's u_hat has a nonzero mode with kz > kc (one
+  // band <- no realization's u_hat has a nonzero mode outside the mask (one
   // synchronisation, when the state is set)
   int check_band() {
     band = false;
@@ -698,7 +699,7 @@ struct Engine final : EngineBase {
     LD_LAUNCHED();
     for (int b = 0; b < B; ++b) {
       hipLaunchKernelGGL((k_band_check<T, N>), dim3(1024), dim3(256), 0, st, (const C<T>*)at(uk, b),
-                         (size_t)3 * N * N, kc, bandflag);
+                         (size_t)3 * CF, sphere(), bandflag);
       LD_LAUNCHED();
     }
     int h = 1;

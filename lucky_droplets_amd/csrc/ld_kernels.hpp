@@ -638,20 +638,22 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
   for (int n2 = 0; n2 < N2; ++n2) st32(Ou, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(u[n2], scale));
 }
 
-// flag[0] <- 1 if any of the `lines` spectral lines of u (Nf entries each) has
-// a nonzero (or NaN) entry with kz > kc; flag[0] is cleared beforehand.  Those
-// modes are outside the dealiasing mask, and a state without them keeps none
-// through the RK4 stages and the step epilogue.
+// flag[0] <- 1 if any of the n spectral entries of u ([f][kx][ky][kz] layout)
+// with |k|^2 > r2 is nonzero (or NaN); flag[0] is cleared beforehand.  With r2
+// = kc (kc + 1) those are the modes outside the dealiasing mask, and a state
+// without them keeps none through the RK4 stages and the step epilogue.
 template <typename T, int N>
-__global__ void __launch_bounds__(256) k_band_check(const C<T>* __restrict__ u, size_t lines, int kc, int* flag) {
+__global__ void __launch_bounds__(256) k_band_check(const C<T>* __restrict__ u, size_t n, int r2, int* flag) {
   constexpr int Nf = N / 2 + 1;
-  const int w = Nf - 1 - kc;
-  const size_t tot = lines * (size_t)w;
   bool any = false;
-  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (size_t)gridDim.x * 256) {
-    const size_t line = e / (size_t)w;
-    const C<T> v = u[line * Nf + (size_t)(kc + 1) + e % (size_t)w];
-    any = any || !(v.x == (T)0 && v.y == (T)0);
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+    const int kz = (int)(e % Nf);
+    const size_t line = e / Nf;
+    const int ky = kwave((int)(line % N), N), kx = kwave((int)((line / N) % N), N);
+    if (kx * kx + ky * ky + kz * kz > r2) {
+      const C<T> v = u[e];
+      any = any || !(v.x == (T)0 && v.y == (T)0);
+    }
   }
   if (__syncthreads_or(any) && threadIdx.x == 0) flag[0] = 1;
 }

@@ -423,9 +423,10 @@ def test_dealias_truncation_keeps_results(gpu, N, prec, outside, monkeypatch):
     """The passes that skip the kz > N/3 lines (LD_TRUNCATE=1, the default)
     give the same values as running every line (0): those lines carry only
     modes that the dealiasing mask removes (the nonlinear term) or that are
-    zero (v_hat, and u_hat when the state has none there).  With `outside`
-    the state carries small modes at kz > N/3, which the state check must see
-    (the u, omega transforms then run every line).  Compared as values: a
+    zero (v_hat, and u_hat when the state has no mode outside the mask).
+    With `outside` the state carries small modes outside the mask at
+    kz <= N/3, which the state check must see (the u, omega transforms then
+    move every entry).  Compared as values: a
     skipped zero may differ in sign."""
     import torch
     from lucky_droplets_amd import ic
@@ -438,10 +439,14 @@ def test_dealias_truncation_keeps_results(gpu, N, prec, outside, monkeypatch):
     coord, _ = ic.initial_droplets(cfg, rng)
     uk0 = uk
     if outside:
+        # small modes outside the mask only at kz <= N/3 (the lines the y passes keep)
         kc = N // 3
-        sh = uk[:, :, :, kc + 1:].shape
+        k = np.fft.fftfreq(N, 1.0 / N).astype(int)
+        k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + np.arange(N // 2 + 1)[None, None, :] ** 2
+        sel = (k2 > kc * (kc + 1)) & (np.arange(N // 2 + 1)[None, None, :] <= kc)
         uk = uk.copy()
-        uk[:, :, :, kc + 1:] = 1e-3 * np.abs(uk).max() * (rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+        noise = rng.standard_normal(uk.shape) + 1j * rng.standard_normal(uk.shape)
+        uk = np.where(sel[None], 1e-3 * np.abs(uk).max() * noise, uk)
     outs = []
     for trunc in ("0", "1"):
         monkeypatch.setenv("LD_TRUNCATE", trunc)
