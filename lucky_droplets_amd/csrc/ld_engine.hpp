@@ -343,10 +343,10 @@ struct Engine final : EngineBase {
   }
   // z pass over the (x, y) lines of x-planes [x0, x0 + nx).
   template <int P, int FIN, int FOUT, class Op> int zrun(hipStream_t s, const Op& op, int kid = -2, int x0 = 0,
-                                                         int nx = N) {
+                                                         int nx = N, int kcout = N / 2) {
     LD_TRY(prof_mark(kid, s));
     hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(nx * N / P, 1, B), dim3(ZThreads<N, P, FIN, FOUT>::v), 0,
-                       s, op, (const C<T>*)tw, x0 * N, rs);
+                       s, op, (const C<T>*)tw, x0 * N, rs, kcout);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
@@ -479,7 +479,7 @@ struct Engine final : EngineBase {
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx, kcu, kcu < N / 2 ? sphere() : -1)));
-      LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx)));
+      LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc)));
       LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx, kc, sphere())));
     }
     return LD_OK;

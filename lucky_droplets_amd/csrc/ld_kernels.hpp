@@ -296,10 +296,11 @@ template <typename T> struct alignas(2 * sizeof(C<T>)) C2 { C<T> a, b; };
 // P*Nf entries per field, kept in LDS in the same order (the regions of
 // consecutive line pairs are adjacent), so they move as pairs of entries;
 // line0 * Nf is even (P and the line offset are even), so every pair is
-// aligned.
+// aligned.  Output pairs whose two entries both have kz > kcout are not
+// written (no later pass reads them).
 template <typename T, int N, int P, int FIN, int FOUT, class Op>
 __global__ void __launch_bounds__((ZThreads<N, P, FIN, FOUT>::v))
-k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
+k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcout) {
   static_assert(P % 2 == 0, "pairs of lines");
   const Op op = op0.shifted(rs);
   constexpr int Nf = N / 2 + 1;
@@ -325,9 +326,11 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs) {
     zForward<T, N, P, FOUT>(tile, tw);
     for (int e = threadIdx.x; e < FOUT * (NE / 2); e += nt) {
       const int f = e / (NE / 2), q = 2 * (e % (NE / 2));
+      const int za = q % Nf, zb = (q + 1) % Nf;
+      if (za > kcout && zb > kcout) continue;
       C2<T> v;
-      v.a = zSplit<T, N, P>(tile, f, q / Nf, q % Nf);
-      v.b = zSplit<T, N, P>(tile, f, (q + 1) / Nf, (q + 1) % Nf);
+      v.a = zSplit<T, N, P>(tile, f, q / Nf, za);
+      v.b = zSplit<T, N, P>(tile, f, (q + 1) / Nf, zb);
       reinterpret_cast<C2<T>*>(op.specOut(f) + g0)[q / 2] = v;
     }
   }
@@ -750,8 +753,13 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
 #pragma unroll
   for (int r = 0; r < N2; ++r) a[r] = Lf[posOut<N>(n1, r)];
   lineInv<T, N>(a, Lf, n1, true, tw);
+  // v_hat x-inverse: lines kz <= kcut outside the mask radius are zero and the
+  // next y pass reads none of their entries; the kz > kcut lines (zero too)
+  // are read as they are by the z pass
+  if (live || iz > A.kcut) {
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
+    for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
+  }
 }
 
 // Step epilogue in spectral space (forced-dns-sm.py:597-610, 618): the
