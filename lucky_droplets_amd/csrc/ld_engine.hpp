@@ -465,10 +465,11 @@ struct Engine final : EngineBase {
     LD_TRY(nonlinear_x(s, usp));
     return nonlinear_yz(s, uphys, g);
   }
-  int nonlinear_x(hipStream_t s, const C<T>* usp) {
+  // kcu < N/2: the stage u_hat has no nonzero mode outside the mask
+  int nonlinear_x(hipStream_t s, const C<T>* usp, int kcu = N / 2) {
     LD_TRY(prof_mark(LD_K_XUW, s));
     hipLaunchKernelGGL((k_x_uw3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, s, usp, w6, CF, (const C<T>*)tw,
-                       inv(), rs);
+                       inv(), rs, kcu);
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XUW, s));
     return LD_OK;
@@ -541,7 +542,7 @@ struct Engine final : EngineBase {
   int stage_a(int s) {
     const C<T>* usp = (s == 0) ? uk : us;
     LD_TRY(shell(st, usp, p.isforcing));
-    return nonlinear_x(st, usp);
+    return nonlinear_x(st, usp, band ? kc : N / 2);
   }
   int stage_b(int s, double h) {
     const C<T>* usp = (s == 0) ? uk : us;

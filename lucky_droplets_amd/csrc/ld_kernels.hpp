@@ -596,7 +596,7 @@ template <typename T, int N, int M0> struct XM3 {
 template <typename T, int N, int M>
 __global__ void __launch_bounds__(3 * M * Plan<N>::N1)
 k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const C<T>* __restrict__ tw, T scale,
-        size_t rs) {
+        size_t rs, int kcut) {
   const C<T>* __restrict__ uk = rsel1(uk0, rs);
   C<T>* __restrict__ out = rsel1(out0, rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
@@ -607,12 +607,22 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
   const T ky = (T)kwave(q / Nf, N), kz = (T)(q % Nf);
   C<T>* L[3] = {lds + m * (N + 1), lds + (M + m) * (N + 1), lds + (2 * M + m) * (N + 1)};
   C<T>* Lf = lds + (f * M + m) * (N + 1);
+  // kcut < N/2: u_hat has no mode outside the mask, so a line with
+  // ky^2 + kz^2 > kcut (kcut + 1) is zero (not read); its outputs are written
+  // only where a later pass reads them as they are (kz > kcut: the z pass)
+  const int kyi = kwave(q / Nf, N), kzi = q % Nf;
+  const bool live = kcut >= N / 2 || kyi * kyi + kzi * kzi <= kcut * (kcut + 1);
+  const bool wr = live || kzi > kcut;
   C<T> u[N2], w[N2];
+  if (live) {
 #pragma unroll
-  for (int r = 0; r < N2; ++r) {
-    u[r] = ld32(uk + f * fs, (uint32_t)(q + posOut<N>(n1, r) * js));
-    Lf[posOut<N>(n1, r)] = u[r];
+    for (int r = 0; r < N2; ++r) u[r] = ld32(uk + f * fs, (uint32_t)(q + posOut<N>(n1, r) * js));
+  } else {
+#pragma unroll
+    for (int r = 0; r < N2; ++r) u[r] = mkc<T>((T)0, (T)0);
   }
+#pragma unroll
+  for (int r = 0; r < N2; ++r) Lf[posOut<N>(n1, r)] = u[r];
   __syncthreads();
   // vorticity component f = i (k x u)_f  (forced-dns-sm.py:294-296)
 #pragma unroll
@@ -633,12 +643,16 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
   // (lineInv synchronises before it reuses the regions)
   lineInv<T, N>(w, Lf, n1, true, tw);
   C<T>* Ow = out + (3 + f) * fs;
+  if (wr) {
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) st32(Ow, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(w[n2], scale));
+    for (int n2 = 0; n2 < N2; ++n2) st32(Ow, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(w[n2], scale));
+  }
   lineInv<T, N>(u, Lf, n1, true, tw);
   C<T>* Ou = out + f * fs;
+  if (wr) {
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) st32(Ou, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(u[n2], scale));
+    for (int n2 = 0; n2 < N2; ++n2) st32(Ou, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(u[n2], scale));
+  }
 }
 
 // flag[0] <- 1 if any of the n spectral entries of u ([f][kx][ky][kz] layout)
