@@ -495,7 +495,7 @@ struct Engine final : EngineBase {
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, w6, w6, 3, inv(), x0, nx, kc, sphere())));
       if (x0 == 0 && wait) LD_HIP(hipStreamWaitEvent(s, wait, 0));
-      LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx)));
+      LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc)));
     }
     return LD_OK;
   }
@@ -507,16 +507,16 @@ struct Engine final : EngineBase {
     OpZFwd<T, N, P1, 1> of{fcin, wn + 3 * CF, RF, CF};
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
-      LD_TRY((zrun<P1, 0, 1>(s, of, -2, x0, nx)));
-      LD_TRY((pass<1, -1>(s, wn, wn, 4, (T)1, x0, nx)));
+      LD_TRY((zrun<P1, 0, 1>(s, of, -2, x0, nx, kc)));
+      LD_TRY((pass<1, -1>(s, wn, wn, 4, (T)1, x0, nx, kc, sphere())));
     }
     LD_TRY(prof_mark(LD_K_XKN, s));
     hipLaunchKernelGGL((k_x_kn4<T, N, M4>), dim3(NBLK_X4, 1, B), dim3(4 * M4 * N1), 0, s, wn, CF, (const C<T>*)tw,
-                       inv(), rs);
+                       inv(), rs, kc);
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XKN, s));
     for (int x0 = 0; x0 < N; x0 += nx) {
-      LD_TRY((pass<1, +1>(s, wn, wn, 1, inv(), x0, nx)));
+      LD_TRY((pass<1, +1>(s, wn, wn, 1, inv(), x0, nx, kc, sphere())));
       if (kz) LD_TRY((zrun<P1, 1, 0>(s, *kz, LD_K_ZKN, x0, nx)));
     }
     return LD_OK;

@@ -858,7 +858,7 @@ template <typename T, int N, int M0> struct XM4 {
 };
 template <typename T, int N, int M>
 __global__ void __launch_bounds__(4 * M * Plan<N>::N1)
-k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs) {
+k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs, int kcut) {
   C<T>* w = rsel1(w0, rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
   __shared__ C<T> lds[4 * M * (N + 1)];
@@ -871,10 +871,20 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs) {
   C<T>* L2 = lds + (2 * M + m) * (N + 1);
   C<T>* L3 = lds + (3 * M + m) * (N + 1);
   C<T>* Lg = lds + (g * M + m) * (N + 1);
+  // kcut < N/2: kn_hat is masked, so lines with ky^2 + kz^2 > kcut (kcut + 1)
+  // read no input (the previous y pass wrote none there) and give zero; of
+  // those, only the kz > kcut lines are written (the z pass reads them)
+  const int kyl = kwave(iy, N);
+  const bool live = kcut >= N / 2 || kyl * kyl + iz * iz <= kcut * (kcut + 1);
   C<T> a[N2];
   const C<T>* W = w + g * fs;
+  if (live) {
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(W, (uint32_t)(q + posIn<N>(n1, n2) * js));
+    for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(W, (uint32_t)(q + posIn<N>(n1, n2) * js));
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) a[n2] = mkc<T>((T)0, (T)0);
+  }
   lineFwd<T, N>(a, Lg, n1, true, tw);
   __syncthreads();
 #pragma unroll
@@ -895,7 +905,7 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs) {
     for (int r = 0; r < N2; ++r) a[r] = L0[posOut<N>(n1, r)];
   }
   lineInv<T, N>(a, L0, n1, g == 0, tw);
-  if (g == 0) {
+  if (g == 0 && (live || iz > kcut)) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) st32(w, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
   }
