@@ -555,7 +555,7 @@ struct Engine final : EngineBase {
     ra.nl = w6; ra.us = usp; ra.ubase = uk; ra.acc = acc; ra.usnext = (s < 3) ? us : nullptr; ra.ku = nullptr;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
     ra.wacc = (T)wh; ra.anext = (T)anext; ra.scale = inv(); ra.acc_init = (s == 0);
-    ra.kcut = kc;
+    ra.kcut = kc; ra.band = band ? 1 : 0;
     LD_TRY(prof_mark(LD_K_XRHS, st));
     hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw, rs);
     LD_LAUNCHED();
@@ -707,6 +707,13 @@ struct Engine final : EngineBase {
     LD_HIP(hipMemcpyAsync(&h, bandflag, sizeof(int), hipMemcpyDeviceToHost, st));
     LD_HIP(hipStreamSynchronize(st));
     band = h == 0;
+    // the RK update skips the lines outside the mask radius of such a state:
+    // its stage input and accumulator start (and stay) zero there
+    if (band) {
+      const C<T> z = mkc<T>((T)0, (T)0);
+      LD_TRY(fill<C<T>>(st, acc, 3 * CF, z));
+      LD_TRY(fill<C<T>>(st, us, 3 * CF, z));
+    }
     return LD_OK;
   }
   int state_get(void* ukout, void* nout, double* coord) override {
@@ -879,7 +886,7 @@ struct Engine final : EngineBase {
     ra.nl = w6; ra.us = ukp; ra.ubase = ukp; ra.acc = nullptr; ra.usnext = nullptr; ra.ku = (C<T>*)ku;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit * visc); ra.tps = (T)p.tps;
     ra.wacc = (T)0; ra.anext = (T)0; ra.scale = inv(); ra.acc_init = 0;
-    ra.kcut = kc;
+    ra.kcut = kc; ra.band = 0;
     hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw, rs);
     LD_LAUNCHED();
     // the physical-space v the droplets see uses visc as well (forced-dns-sm.py:319-321)

@@ -573,6 +573,8 @@ template <typename T> struct RhsArgs {
   int kcut;            // kcut < N/2: lines (ky, kz) with ky^2 + kz^2 > kcut (kcut + 1) hold no mode
                        // inside the mask, so their nonlinear term is not read (those entries of nl
                        // may hold data of an earlier pass)
+  int band;            // 1: us, ubase and acc are zero on every line outside the mask radius, so
+                       // those lines need no RK update (their zero outputs are already in place)
   __device__ RhsArgs shifted(size_t rs) const {
     RhsArgs a = *this;
     a.nl = rsel1(nl, rs); a.us = rsel1(us, rs); a.ubase = rsel1(ubase, rs); a.acc = rsel(acc, rs);
@@ -715,8 +717,10 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
   __syncthreads();
   const C<T>* US0 = A.us; const C<T>* US1 = A.us + fs; const C<T>* US2 = A.us + 2 * fs;
   const C<T>* UB0 = A.ubase; const C<T>* UB1 = A.ubase + fs; const C<T>* UB2 = A.ubase + 2 * fs;
+  const bool idle = A.band && !live;
 #pragma unroll 2
   for (int r = f; r < N2; r += 3) {
+    if (idle) break;
     const int ix = posOut<N>(n1, r);
     const Mode<T> md = mode<T, N>(ix, iy, iz);
     const uint32_t o = (uint32_t)(q + ix * js);
