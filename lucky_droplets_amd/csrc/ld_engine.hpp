@@ -644,7 +644,8 @@ struct Engine final : EngineBase {
                        1, rs);
     LD_LAUNCHED();
     // reality round trip + projection + blow-up maxima in one spectral pass
-    hipLaunchKernelGGL((k_epi_proj<T, N>), dim3(NBLK_X, 1, B), dim3(256), 0, st, (const C<T>*)acc, uk, CF, maxpart, rs);
+    hipLaunchKernelGGL((k_epi_proj<T, N>), dim3(NBLK_X, 1, B), dim3(256), 0, st, (const C<T>*)acc, uk, CF, maxpart, rs,
+                       band ? sphere() : -1);
     LD_LAUNCHED();
     hipLaunchKernelGGL(k_max_final, dim3(1, 1, B), dim3(256), 0, st, (const double*)maxpart, NBLK_X,
                        100.0 * (double)N * N * N, flag, lastmax, rs);

@@ -789,10 +789,12 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
 // result directly, so it does not add the round trip's transform rounding.
 // Then u_hat += k (k . u_hat) mask / lap, and per-block maxima of Re(u_hat)
 // for the blow-up test.  Blocks take runs of entry pairs grid-stride, two
-// pairs of each field per thread in flight.
+// pairs of each field per thread in flight.  r2 >= 0: acc and uk are zero at
+// every mode with |k|^2 > r2, so pairs of two such modes are skipped (they
+// add 0 to the maximum).
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
-k_epi_proj(const C<T>* acc0, C<T>* uk0, size_t fs, double* maxpart0, size_t rs) {
+k_epi_proj(const C<T>* acc0, C<T>* uk0, size_t fs, double* maxpart0, size_t rs, int r2) {
   const C<T>* acc = rsel1(acc0, rs);
   C<T>* uk = rsel1(uk0, rs);
   double* maxpart = rsel1(maxpart0, rs);
@@ -803,12 +805,20 @@ k_epi_proj(const C<T>* acc0, C<T>* uk0, size_t fs, double* maxpart0, size_t rs) 
   double mx = -INFINITY;
   const uint32_t stride = gridDim.x * 256u;
   // a thread takes entry pairs (2 e2, 2 e2 + 1): 16-byte FP32 accesses
+  auto inside = [&](uint32_t e) {
+    const int iz = (int)(e % Nf);
+    const uint32_t line = e / Nf;
+    const int ky = kwave((int)(line & msk), N), kx = kwave((int)(line / N), N);
+    return kx * kx + ky * ky + iz * iz <= r2;
+  };
   for (uint32_t e0 = blockIdx.x * 256u + threadIdx.x; e0 < NE2; e0 += U * stride) {
     C2<T> a[U][3];
+    bool kp[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t e2 = e0 + u * stride;
-      if (e2 < NE2) {
+      kp[u] = e2 < NE2 && (r2 < 0 || inside(2 * e2) || inside(2 * e2 + 1));
+      if (kp[u]) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) a[u][c] = ld32(reinterpret_cast<const C2<T>*>(acc + c * fs), e2);
       }
@@ -817,6 +827,7 @@ k_epi_proj(const C<T>* acc0, C<T>* uk0, size_t fs, double* maxpart0, size_t rs) 
     for (int u = 0; u < U; ++u) {
       const uint32_t e2 = e0 + u * stride;
       if (e2 >= NE2) break;
+      if (!kp[u]) { mx = fmax(mx, 0.0); continue; }
       C2<T> o[3];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
