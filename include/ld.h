@@ -23,7 +23,10 @@
  *      vector field    3 fields back to back, component-major
  *      droplet rows    (Np, 7) float64    [x, y, z, vx, vy, vz, m/M0]
  *  - Forward transforms are unnormalised; inverse transforms carry 1/N^3.
- *  - Every call is asynchronous on the context's stream (ld_set_stream).
+ *  - Every call is asynchronous on the context's stream (ld_set_stream),
+ *    except where noted: ld_state_set (and ld_rk4 / ld_step, which set the
+ *    state) waits once for its check of the spectral state, and the stepping
+ *    calls wait to read the blow-up flag.
  *  - Return value: LD_OK or an LD_ERR_* code; ld_last_error() gives text.
  *  - Contexts are independent; a context must not be used from two threads
  *    at once.
@@ -122,7 +125,9 @@ int ld_step(ld_ctx* ctx, double h, void* uk, void* n, double* coord, int64_t np,
  * (B, 3, N, N, N/2+1), n (B, N, N, N), coord (B, np, 7) with np rows per
  * realization; ld_advance steps all of them with one launch per kernel
  * (blockIdx.z = realization).  Each realization evolves exactly as it would
- * alone (same kernels, same arithmetic). */
+ * alone (same kernels, same arithmetic).  ld_state_set also checks whether
+ * uk has any nonzero mode outside the dealiasing mask (one synchronisation);
+ * without one, the steps skip the work whose inputs are zero there. */
 int ld_state_set(ld_ctx* ctx, const void* uk, const void* n, const double* coord, int64_t np);
 int ld_state_get(ld_ctx* ctx, void* uk, void* n, double* coord);
 int ld_advance(ld_ctx* ctx, double h, int32_t nsteps);
