@@ -1005,6 +1005,7 @@ struct Engine final : EngineBase {
   }
 
   int advance_plain(double h, int nsteps) override {
+    band = false;   // until the check at the end (an early error leaves it off)
     LD_TRY(plain_alloc());
     LD_TRY(join(st, st2, evB));
     const size_t nr = 7 * (size_t)np;
