@@ -509,6 +509,46 @@ def test_ensemble_matches_single_runs(gpu, prec):
         d.close()
 
 
+def test_ensemble_truncation_keeps_results(gpu, monkeypatch):
+    """A batched ensemble whose realizations alternate between a state with no
+    mode outside the mask and one with such modes: the state check covers every
+    realization, the zeroed RK buffers are per realization, and the steps give
+    the same values as with the truncation off (graph replay, set twice)."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = preset("c1").replace(precision="f32", N=32, n_particles=2000)
+    B, steps = 3, 3
+    uks, ns, cs, dts = [], [], [], []
+    for b in range(B):
+        rng = np.random.default_rng(200 + b)
+        uk, n, u = ic.initial_flow(cfg, rng)
+        dts.append(ic.cfl_dt(cfg, u))
+        coord, _ = ic.initial_droplets(cfg, rng)
+        uks.append(uk); ns.append(n); cs.append(coord)
+    dt = min(dts)
+    outside = np.stack(uks).copy()
+    outside[1, :, 10, 10, 2] = 1e-3 * np.abs(outside).max()   # |k|^2 = 204 > 10 * 11: outside the mask
+    outs = []
+    for trunc in ("0", "1"):
+        monkeypatch.setenv("LD_TRUNCATE", trunc)
+        e = ForcedDNS(cfg, batch=B, graph=True)
+        res = []
+        for start in (np.stack(uks), outside):
+            t = (_t(start, gpu, e.cdtype), _t(np.stack(ns), gpu, e.rdtype), _t(np.stack(cs), gpu, torch.float64))
+            e.set_state(*t)
+            e.advance(dt, steps)
+            e.get_state(*t)
+            torch.cuda.synchronize()
+            res.append([x.cpu().numpy() for x in t])
+        e.close()
+        outs.append(res)
+    for r0, r1 in zip(outs[0], outs[1]):
+        for a, b in zip(r0, r1):
+            assert np.array_equal(a, b)
+
+
 def test_metric_workload_fp32_against_fp64_and_invariants(gpu):
     """The metric workload itself (256^3 FP32 fields + 10^6 droplets), where the
     CPU oracle is too slow: two fused steps in FP32 against the same steps of the
