@@ -203,6 +203,95 @@ def test_rhs_divergence_free_and_density_flux():
     assert np.max(np.abs(kn0)) < 1e-10
 
 
+def test_slow_manifold_velocity_beltrami_closed_form():
+    """ABC flow (|k| = 1, p = 0): v_hat = (u_hat - tps (0 - nu k^2 u_hat)) mask
+    = (1 + tps nu) u_hat, and v_z is shifted by -tps g (forced-dns-sm.py:319-321).
+    A flipped sign of the tps term gives (1 - tps nu); a dropped shift leaves
+    mean(v_z) = 0."""
+    N = 16
+    nu, tps, g = 0.05, 0.4, 2.5
+    fl = oracle.Flow(N, nu, [1], 1.0, tps=tps, g=g, isforcing=False)
+    G = fl.grid
+    u = abc_field(N)
+    uk = np.stack([G.rfft_mpi(u[c]) for c in range(3)])
+    fl.full_RHS(0, uk, np.ones((N, N, N)), np.zeros((0, 7)), np.zeros((0, 7)), None)
+    expect = (1 + tps * nu) * u
+    expect[2] -= tps * g
+    assert relerr(fl.last["v"], expect) < 1e-13
+    # forcing on: f_hat = F u_hat is solenoidal, still no pressure; v keeps the same form
+    fl1 = oracle.Flow(N, nu, [1], 0.7, tps=tps, g=g, isforcing=True)
+    fl1.full_RHS(0, uk, np.ones((N, N, N)), np.zeros((0, 7)), np.zeros((0, 7)), None)
+    assert relerr(fl1.last["v"], expect) < 1e-13
+
+
+def test_slow_manifold_velocity_taylor_green_pressure():
+    """2-D Taylor-Green u = (sin x cos y, -cos x sin y, 0): u x w = -grad(phi),
+    phi = sin^2 x sin^2 y, so the pressure is p_hat = -phi_hat and
+    v = (1 + 2 tps nu) u - tps grad(phi) - tps g e_z (forced-dns-sm.py:309, 319-321).
+    Pins the sign of the pressure term inside v (|k|^2 = 2 for u)."""
+    N = 16
+    nu, tps, g = 0.03, 0.25, 1.5
+    fl = oracle.Flow(N, nu, [1], 1.0, tps=tps, g=g, isforcing=False)
+    G = fl.grid
+    x, y, z = grid_xyz(N)
+    u = np.stack([np.sin(x) * np.cos(y), -np.cos(x) * np.sin(y), 0 * x])
+    uk = np.stack([G.rfft_mpi(u[c]) for c in range(3)])
+    fl.full_RHS(0, uk, np.ones((N, N, N)), np.zeros((0, 7)), np.zeros((0, 7)), None)
+    grad_phi = np.stack([np.sin(2 * x) * np.sin(y) ** 2, np.sin(x) ** 2 * np.sin(2 * y), 0 * x])
+    expect = (1 + 2 * tps * nu) * u - tps * grad_phi
+    expect[2] -= tps * g
+    assert relerr(fl.last["v"], expect) < 1e-13
+    # the pressure term is not negligible here: dropping it misses by O(tps)
+    assert np.max(np.abs(tps * grad_phi)) > 0.1
+
+
+def _droplet_flow(N, nparts, seed, tps=0.02, g=0.8, u_scale=1.0):
+    cfg = Config(N=N, nu=0.05, nshells=2, profile="k4", n_particles=nparts, st_dist=("uniform", 0.05, 1.0),
+                 gravity=True)
+    fl = oracle.Flow(N, 0.05, [1, 2], cfg.f0, tps=tps, g=g)
+    rng = np.random.default_rng(seed)
+    uk, n, _ = ic.initial_flow(cfg, rng)
+    coord, pid = ic.initial_droplets(cfg, rng)
+    n = n + 0.6 * (rng.random(n.shape) - 0.5)
+    dr = oracle.Droplets(cfg.L, N, coord.copy(), cfg.sts, g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0,
+                         pid.reshape(-1, 1).copy())
+    return fl, dr, uk * u_scale, n, coord
+
+
+def test_density_rhs_mean_equals_minus_deposit_mean():
+    """The flux term -i k.(n v)_hat has no k = 0 mode, so the mean of kn is
+    -mean(fc) (forced-dns-sm.py:325-330): the droplets' growth is a sink of n."""
+    fl, dr, uk, n, coord = _droplet_flow(16, 300, 4)
+    ku, kn, kp, _ = fl.full_RHS(0, uk, n, coord.copy(), coord.copy(), dr)
+    fc = fl.last["fc"]
+    assert np.sum(fc) > 0
+    assert abs(np.mean(kn) + np.mean(fc)) < 1e-12 * np.mean(fc)
+    # and the sink equals the droplets' mass gain: sum fc dx^3 = sum W = sum kp[:, 6]
+    G = fl.grid
+    assert abs(np.sum(fc) * G.dx ** 3 - np.sum(kp[:, 6])) < 1e-12 * np.sum(kp[:, 6])
+
+
+def test_density_rhs_still_flow_single_droplet():
+    """u = 0, uniform n = n0, one droplet: v = -tps g e_z is constant, so n v has
+    no flux and kn = -irfft(rfft(fc) mask) exactly; W = growth (1 + sqrt(sts/St))^2
+    n0 |v_p - v| (particles.py:828), sum fc dx^3 = W (forced-dns-sm.py:325-330)."""
+    N = 16
+    tps, g, n0 = 0.02, 0.8, 1.7
+    fl, dr, uk, n, coord = _droplet_flow(N, 1, 6, tps=tps, g=g, u_scale=0.0)
+    n = np.full((N, N, N), n0)
+    ku, kn, kp, _ = fl.full_RHS(0, uk, n, coord.copy(), coord.copy(), dr)
+    G = fl.grid
+    fc = fl.last["fc"]
+    st = (coord[0, 6] / dr.factor) ** (2 / 3)
+    vrel = coord[0, 3:6] - np.array([0.0, 0.0, -tps * g])
+    W = dr.growthfactor * (1 + np.sqrt(dr.st_s / st)) ** 2 * n0 * np.linalg.norm(vrel)
+    assert abs(kp[0, 6] - W) < 1e-13 * W
+    assert abs(np.sum(fc) * G.dx ** 3 - W) < 1e-12 * W
+    expect = -G.irfft_mpi(G.rfft_mpi(fc) * G.dealias)
+    assert relerr(kn, expect) < 1e-12
+    assert np.max(np.abs(kn)) > 0
+
+
 def test_reality_round_trip_symmetrises_planes():
     """rfft(irfft(X)) makes the kz = 0 and kz = N/2 planes Hermitian and
     leaves other modes unchanged (forced-dns-sm.py:597-603)."""
