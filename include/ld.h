@@ -189,6 +189,19 @@ int ld_spectrum(ld_ctx* ctx, const void* f, int32_t nfields, double weight, doub
  * cumulated from the highest shell down (forced-dns-sm.py:426-428); ku from
  * ld_full_rhs with visc = 0, forc = 0 (:425).  out[N/2+1] device double. */
 int ld_flux(ld_ctx* ctx, const void* uk, const void* ku, double* out);
+/* Dissipation spectrum: e3d_to_e1d(nu |k|^2 sum_c |u_c|^2 * normalize) of the
+ * spectral velocity uk into out[N/2+1] (device double); its sum is the
+ * dissipation rate 2 nu sum_k k^2 E(k).  The reference leaves its dissipation
+ * diagnostic commented out (forced-dns-sm.py:473); this is the standard one. */
+int ld_dissipation(ld_ctx* ctx, const void* uk, double* out);
+/* Physical divergence field div = diff_x(u) + diff_y(v) + diff_z(w) of the
+ * spectral velocity uk (one real field out): the reference's spectral
+ * derivatives forced-dns-sm.py:216-229, whose max |div| save() prints (:472). */
+int ld_divergence(ld_ctx* ctx, const void* uk, void* div);
+/* Droplets of the resident state (ld_state_set / ld_advance): sum, min, max
+ * and sum of squares of the mass column m/M0 -> out[4] (device double); the
+ * total droplet mass that plot.py:56-72 tracks over a run. */
+int ld_droplet_stats(ld_ctx* ctx, double* out);
 /* sum, min, max and sum of squares over nfields real fields -> out[4]
  * (device double): energy, n mean/min/max (forced-dns-sm.py:467-471). */
 int ld_field_stats(ld_ctx* ctx, const void* x, int32_t nfields, double* out);

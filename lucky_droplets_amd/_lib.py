@@ -96,6 +96,9 @@ _SIGS = {
     "ld_spectrum": [_VP, _VP, ctypes.c_int32, ctypes.c_double, _VP],
     "ld_flux": [_VP, _VP, _VP, _VP],
     "ld_field_stats": [_VP, _VP, ctypes.c_int32, _VP],
+    "ld_dissipation": [_VP, _VP, _VP],
+    "ld_divergence": [_VP, _VP, _VP],
+    "ld_droplet_stats": [_VP, _VP],
     "ld_collide_pairs": [_VP, _VP, _VP, ctypes.c_int64, ctypes.c_double, _VP, ctypes.c_int64,
                          ctypes.POINTER(ctypes.c_int64)],
     "ld_coalesce": [_VP, _VP, ctypes.c_int64, _VP, ctypes.c_int64, _VP, ctypes.POINTER(ctypes.c_int64)],

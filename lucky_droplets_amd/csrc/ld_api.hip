@@ -238,6 +238,21 @@ int ld_flux(ld_ctx* ctx, const void* uk, const void* ku, double* out) {
   return ctx->eng->flux(uk, ku, out);
 }
 
+int ld_dissipation(ld_ctx* ctx, const void* uk, double* out) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(uk); LD_NONNULL(out);
+  return ctx->eng->dissipation(uk, out);
+}
+
+int ld_divergence(ld_ctx* ctx, const void* uk, void* div) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(uk); LD_FIELD(div);
+  return ctx->eng->divergence(uk, div);
+}
+
+int ld_droplet_stats(ld_ctx* ctx, double* out) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(out);
+  return ctx->eng->droplet_stats(out);
+}
+
 int ld_field_stats(ld_ctx* ctx, const void* x, int32_t nfields, double* out) {
   LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(x); LD_NONNULL(out);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }

@@ -20,15 +20,20 @@ template <int N> __device__ __forceinline__ int shellBin(int k2) {
   return b <= N / 2 ? b : -1;
 }
 
-// mode: 0 -> w * sum_c |f_c|^2 * normalize (nf fields); 1 -> Re(conj(a).b) * mask * normalize (3 fields each)
+// mode: 0 -> w * sum_c |f_c|^2 * normalize (nf fields); 1 -> Re(conj(a).b) * mask * normalize (3 fields each);
+// 2 -> w * |k|^2 * sum_c |f_c|^2 * normalize (nf fields: the dissipation spectrum for w = nu)
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
 k_shell_partial(const C<T>* __restrict__ a, const C<T>* __restrict__ b, int nf, size_t fs, int mode, double w,
                 double norm0, double* __restrict__ part) {
   constexpr int Nf = N / 2 + 1;
-  __shared__ double bins[Nf];
-  for (int i = threadIdx.x; i < Nf; i += blockDim.x) bins[i] = 0.0;
+  // one set of bins per wavefront (256 threads = 4), summed in a fixed order at
+  // the end: additions into one bin come only from the lanes of one
+  // instruction, whose order the hardware fixes, so the sums are reproducible
+  __shared__ double bins[4][Nf];
+  for (int i = threadIdx.x; i < 4 * Nf; i += blockDim.x) bins[i / Nf][i % Nf] = 0.0;
   __syncthreads();
+  double* wbins = bins[threadIdx.x >> 6];
   const size_t per = (fs + gridDim.x - 1) / gridDim.x;
   const size_t i0 = blockIdx.x * per, i1 = (i0 + per < fs) ? i0 + per : fs;
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
@@ -40,12 +45,12 @@ k_shell_partial(const C<T>* __restrict__ a, const C<T>* __restrict__ b, int nf, 
     if (s < 0) continue;
     const double nrm = (iz == 0 || iz == N / 2) ? norm0 : 2.0 * norm0;
     double v = 0.0;
-    if (mode == 0) {
+    if (mode != 1) {
       for (int c = 0; c < nf; ++c) {
         const C<T> z = a[c * fs + i];
         v += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
       }
-      v *= w * nrm;
+      v *= (mode == 2 ? w * (double)k2 : w) * nrm;
     } else {
       const int ki = kround(k2);
       if (3 * ki > N) continue;   // dealias mask
@@ -55,10 +60,11 @@ k_shell_partial(const C<T>* __restrict__ a, const C<T>* __restrict__ b, int nf, 
       }
       v *= nrm;
     }
-    atomicAdd(&bins[s], v);
+    atomicAdd(&wbins[s], v);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < Nf; i += blockDim.x) part[(size_t)blockIdx.x * Nf + i] = bins[i];
+  for (int i = threadIdx.x; i < Nf; i += blockDim.x)
+    part[(size_t)blockIdx.x * Nf + i] = ((bins[0][i] + bins[1][i]) + bins[2][i]) + bins[3][i];
 }
 
 // out[s] = sum over blocks (fixed order); cumulate != 0 -> cumulative sum from the highest shell down
@@ -79,6 +85,28 @@ __global__ void k_shell_final(const double* __restrict__ part, int nblk, int cum
     } else {
       for (int s = 0; s < Nf; ++s) out[s] = tot[s];
     }
+  }
+}
+
+// Spectral divergence i (kx u_x + ky u_y + kz u_z) of a vector field, with the
+// derivative of each axis's Nyquist mode taken as zero: the reference's
+// diff_x / diff_y / diff_z (forced-dns-sm.py:216-229) differentiate with a
+// 1-D rfft along the axis, multiply by i k (k = 0..N/2) and irfft, which drops
+// the imaginary Nyquist coefficient that i (N/2) times a real one makes.
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_div_spec(const C<T>* __restrict__ u, size_t fs, C<T>* __restrict__ out) {
+  constexpr int Nf = N / 2 + 1;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < fs; i += (size_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(i % Nf);
+    const size_t rem = i / Nf;
+    const int ix = (int)(rem / N), iy = (int)(rem % N);
+    const T kx = ix == N / 2 ? (T)0 : (T)kwave(ix, N);
+    const T ky = iy == N / 2 ? (T)0 : (T)kwave(iy, N);
+    const T kz = iz == N / 2 ? (T)0 : (T)iz;
+    const C<T> a = u[i], b = u[fs + i], c = u[2 * fs + i];
+    const T re = kx * a.x + ky * b.x + kz * c.x, im = kx * a.y + ky * b.y + kz * c.y;
+    out[i] = mkc<T>(-im, re);      // i (k . u)
   }
 }
 

@@ -1084,6 +1084,45 @@ struct Engine final : EngineBase {
     LD_LAUNCHED();
     return LD_OK;
   }
+  // Dissipation spectrum nu |k|^2 sum_c |u_c|^2 normalize by shell; its sum is
+  // the dissipation rate 2 nu sum_k k^2 E(k).
+  int dissipation(const void* ukin, double* out) override {
+    LD_TRY(diag_alloc());
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_shell_partial<T, N>), dim3(NBLK_DIAG), dim3(256), 0, st, (const C<T>*)ukin,
+                       (const C<T>*)nullptr, 3, CF, 2, p.nu, norm0(), dpart);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL((k_shell_final<N>), dim3(1), dim3(256), 0, st, (const double*)dpart, NBLK_DIAG, 0, out);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  // Physical divergence field div = d_x u_x + d_y u_y + d_z u_z of the
+  // spectral velocity uk (forced-dns-sm.py:216-229, 472); wn is its spectral
+  // work buffer, w6 the transform's.
+  int divergence(const void* ukin, void* divout) override {
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_div_spec<T, N>), dim3(1024), dim3(256), 0, st, (const C<T>*)ukin, CF, wn);
+    LD_LAUNCHED();
+    return irfft(wn, divout, 1);
+  }
+  // Sum, min, max and sum of squares of the resident droplets' mass column
+  // (m/M0; plot.py:56-72 tracks the total).
+  int droplet_stats(double* out) override {
+    LD_TRY(diag_alloc());
+    LD_TRY(join(st, st2, evB));
+    if (np == 0) {
+      const double z[4] = {0.0, 0.0, 0.0, 0.0};
+      LD_HIP(hipMemcpyAsync(out, z, sizeof(z), hipMemcpyHostToDevice, st));
+      LD_HIP(hipStreamSynchronize(st));
+      return LD_OK;
+    }
+    hipLaunchKernelGGL((k_stats_partial<PT>), dim3(NBLK_DIAG), dim3(256), 0, st, (const PT*)(v0 + 3 * cap),
+                       (size_t)np, dpart);
+    LD_LAUNCHED();
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, st, (const double*)dpart, NBLK_DIAG, out);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
   int stats(const void* x, int nf, double* out) override {
     LD_TRY(diag_alloc());
     LD_TRY(join(st, st2, evB));

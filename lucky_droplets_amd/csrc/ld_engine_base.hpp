@@ -38,6 +38,9 @@ struct EngineBase {
                        int64_t* nevents) = 0;
   virtual int spectrum(const void* f, int nf, double w, double* out) = 0;
   virtual int flux(const void* uk, const void* ku, double* out) = 0;
+  virtual int dissipation(const void* uk, double* out) = 0;
+  virtual int divergence(const void* uk, void* div) = 0;
+  virtual int droplet_stats(double* out) = 0;
   virtual int stats(const void* x, int nf, double* out) = 0;
   virtual int profile(int kernel, int enable) = 0;
   virtual int profile_read(double* total_ms, int64_t* launches) = 0;

@@ -298,6 +298,35 @@ class ForcedDNS:
         _lib.check(self.lib.ld_field_stats(self.ctx, _ptr(x), nf, _ptr(out)))
         return tuple(out.cpu().tolist())
 
+    def dissipation(self, uk: torch.Tensor) -> torch.Tensor:
+        """Dissipation spectrum nu |k|^2 sum_c |u_c|^2 normalize by shell (N/2+1,);
+        its sum is the dissipation rate 2 nu sum_k k^2 E(k)."""
+        self._chk(uk, self.cdtype, self.cshape(3), "uk")
+        out = torch.zeros(self.Nf, dtype=torch.float64, device=self.device)
+        self._stream()
+        _lib.check(self.lib.ld_dissipation(self.ctx, _ptr(uk), _ptr(out)))
+        return out
+
+    def divergence(self, uk: torch.Tensor, div: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Physical divergence diff_x(u) + diff_y(v) + diff_z(w) (forced-dns-sm.py:216-229)."""
+        self._chk(uk, self.cdtype, self.cshape(3), "uk")
+        div = self.zeros_real() if div is None else self._chk(div, self.rdtype, self.rshape(), "div")
+        self._stream()
+        _lib.check(self.lib.ld_divergence(self.ctx, _ptr(uk), _ptr(div)))
+        return div
+
+    def divergence_max(self, uk: torch.Tensor) -> float:
+        """max |div u| as save() prints it (forced-dns-sm.py:472)."""
+        _, mn, mx, _ = self.field_stats(self.divergence(uk))
+        return max(abs(mn), abs(mx))
+
+    def droplet_stats(self):
+        """(sum, min, max, sum of squares) of the resident droplets' mass m/M0."""
+        out = torch.zeros(4, dtype=torch.float64, device=self.device)
+        self._stream()
+        _lib.check(self.lib.ld_droplet_stats(self.ctx, _ptr(out)))
+        return tuple(out.cpu().tolist())
+
     def advance_plain(self, h, nsteps: int = 1):
         """nsteps with the reference's unfused dataflow (ld_advance_plain)."""
         self._stream()

@@ -15,7 +15,8 @@ from scipy.fft import fft, fftfreq, ifft, irfft2, rfft2
 
 from .config import Config
 
-_WORKERS = os.cpu_count() or 1
+# scipy.fft threads: the host's cores, at most 16 (one GPU's share of a shared host)
+_WORKERS = min(16, os.cpu_count() or 1)
 
 
 class _Spec:

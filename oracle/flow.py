@@ -21,7 +21,7 @@ unnormalised, inverse transforms carry ``1/N**3``.
 from __future__ import annotations
 
 import numpy as np
-from scipy.fft import fft, ifft, irfft2, rfft2, fftfreq
+from scipy.fft import fft, ifft, irfft, irfft2, rfft, rfft2, fftfreq
 
 # scipy.fft worker count.  The reference passes none (one thread per rank);
 # the CPU baseline may raise it.
@@ -81,6 +81,24 @@ class Grid:
     def energy3d(self, uk: np.ndarray) -> np.ndarray:
         """forced-dns-sm.py:243 (also :423, :722, :740)."""
         return 0.5 * (np.abs(uk[0]) ** 2 + np.abs(uk[1]) ** 2 + np.abs(uk[2]) ** 2) * self.normalize
+
+    # forced-dns-sm.py:216-229 (diff_x, diff_y, diff_z) with num_process == 1:
+    # a 1-D rfft along the axis, times i k with k = 0..N/2, then irfft.
+    def diff(self, u: np.ndarray, axis: int) -> np.ndarray:
+        N = self.N
+        k = np.arange(N // 2 + 1, dtype=np.float64)           # kx_diff / ky_diff / kz_diff (:69-71)
+        shape = [1, 1, 1]
+        shape[axis] = N // 2 + 1
+        return irfft(1j * k.reshape(shape) * rfft(u, axis=axis), N, axis=axis)
+
+    def divergence(self, u: np.ndarray) -> np.ndarray:
+        """diff_x(u) + diff_y(v) + diff_z(w) of a physical velocity (:472)."""
+        return self.diff(u[0], 0) + self.diff(u[1], 1) + self.diff(u[2], 2)
+
+    def dissipation_spectrum(self, uk: np.ndarray, nu: float) -> np.ndarray:
+        """e3d_to_e1d(nu |k|^2 sum_c |u_c|^2 normalize): its sum is 2 nu sum_k k^2 E(k),
+        the dissipation the reference's commented-out line :473 aims at."""
+        return self.e3d_to_e1d(nu * self.k ** 2 * 2.0 * self.energy3d(uk))
 
 
 class Flow:

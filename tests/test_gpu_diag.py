@@ -1,0 +1,74 @@
+"""Diagnostics of save() and plot.py on the device against the oracle:
+dissipation spectrum, divergence field (forced-dns-sm.py:216-229, 472),
+resident droplet mass (plot.py:56-72)."""
+import numpy as np
+import pytest
+
+from helpers import inputs, make_flow_oracle, relerr
+from lucky_droplets_amd.config import Config
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, dev, dtype):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev).to(dtype)
+
+
+def _cfg(N, prec, nparts=0):
+    return Config(N=N, precision=prec, nu=0.02, nshells=2, profile="k4", n_particles=nparts,
+                  st_dist=("uniform", 0.05, 1.0), gravity=True)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_dissipation_and_divergence_match_oracle(gpu, prec):
+    from lucky_droplets_amd.dns import ForcedDNS
+    N = 32
+    cfg = _cfg(N, prec)
+    uk, _, _, _ = inputs(cfg, seed=3)
+    fl = make_flow_oracle(cfg)
+    G = fl.grid
+    d = ForcedDNS(cfg, max_particles=0)
+    tol = 1e-12 if prec == "f64" else 2e-5
+    D = d.dissipation(_t(uk, gpu, d.cdtype)).cpu().numpy()
+    D_ref = G.dissipation_spectrum(uk, cfg.nu_eff)
+    assert relerr(D, D_ref) < tol
+    # a field with divergence: add a gradient part i k phi to the solenoidal one
+    rng = np.random.default_rng(5)
+    phi = (rng.standard_normal(uk.shape[1:]) + 1j * rng.standard_normal(uk.shape[1:])) * G.dealias
+    wk = uk + 1j * np.stack([G.kx, G.ky, G.kz]) * phi * np.abs(uk).max()
+    w = np.stack([G.irfft_mpi(wk[c]) for c in range(3)])
+    div_ref = G.divergence(w)
+    div = d.divergence(_t(wk, gpu, d.cdtype)).cpu().numpy()
+    assert relerr(div, div_ref) < (1e-11 if prec == "f64" else 1e-5)
+    assert abs(d.divergence_max(_t(wk, gpu, d.cdtype)) - np.abs(div_ref).max()) < 1e-5 * np.abs(div_ref).max()
+    # the solenoidal state: divergence at round-off
+    assert d.divergence_max(_t(uk, gpu, d.cdtype)) < (1e-12 if prec == "f64" else 1e-5) * np.abs(div_ref).max()
+    d.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_droplet_mass_of_resident_state(gpu, prec):
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    N, npart = 32, 5000
+    cfg = _cfg(N, prec, npart)
+    uk, n, coord, pid = inputs(cfg, seed=4)
+    d = ForcedDNS(cfg)
+    d.set_state(_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+    s, mn, mx, q = d.droplet_stats()
+    m = coord[:, 6].astype(np.float32 if prec == "f32" else np.float64).astype(np.float64)
+    assert abs(s - m.sum()) < 1e-12 * m.sum()
+    assert mn == m.min() and mx == m.max()
+    assert abs(q - (m * m).sum()) < 1e-12 * (m * m).sum()
+    # after steps the total grows by what the field lost (sum n dx^3 + sum m is invariant)
+    dx3 = cfg.dx ** 3
+    t0 = float(n.sum()) * dx3 + s
+    d.advance(2e-3, 3)
+    s1 = d.droplet_stats()[0]
+    ng = d.zeros_real()
+    d.get_state(None, ng, None)
+    t1 = float(ng.double().sum()) * dx3 + s1
+    assert s1 > s
+    assert abs(t1 - t0) < (1e-9 if prec == "f64" else 1e-3) * (s1 - s)
+    d.close()

@@ -334,3 +334,37 @@ def test_ic_product_equals_oracle_recipe():
     G = fl.grid
     div = G.kx * a[0][0] + G.ky * a[0][1] + G.kz * a[0][2]
     assert np.max(np.abs(div)) < 1e-9 * np.max(np.abs(a[0]))
+
+
+def test_divergence_closed_forms():
+    """diff_x/diff_y/diff_z (forced-dns-sm.py:216-229): the divergence of
+    (sin(2x) cos y, sin z, cos(3y)) is 2 cos(2x) cos y; a solenoidal ABC field
+    has none; the Nyquist mode cos(N x / 2) has zero derivative (the 1-D irfft
+    drops the imaginary Nyquist coefficient)."""
+    N = 16
+    G = oracle.Grid(N)
+    x, y, z = grid_xyz(N)
+    u = np.stack([np.sin(2 * x) * np.cos(y), np.sin(z), np.cos(3 * y)])
+    assert np.max(np.abs(G.divergence(u) - 2 * np.cos(2 * x) * np.cos(y))) < 1e-13
+    # a wrong axis or sign would give -2 cos(2x) cos y or mix in cos(z)
+    assert np.max(np.abs(G.divergence(abc_field(N)))) < 1e-13
+    ny = np.stack([np.cos(N / 2 * x), np.cos(N / 2 * y), np.cos(N / 2 * z)])
+    assert np.max(np.abs(G.divergence(ny))) < 1e-12
+    # diff along each axis separately: d/dy of sin(3y) is 3 cos(3y)
+    assert np.max(np.abs(G.diff(np.sin(3 * y), 1) - 3 * np.cos(3 * y))) < 1e-12
+
+
+def test_dissipation_spectrum_closed_form():
+    """u = (sin(k0 z), 0, 0): energy sum_k E(k) = L^3/4 and dissipation
+    nu int |grad u|^2 = nu k0^2 L^3 / 2, all in shell k0."""
+    N, k0, nu = 16, 3, 0.07
+    G = oracle.Grid(N)
+    x, y, z = grid_xyz(N)
+    u = np.stack([np.sin(k0 * z), 0 * z, 0 * z])
+    uk = np.stack([G.rfft_mpi(u[c]) for c in range(3)])
+    L3 = (2 * np.pi) ** 3
+    E = G.e3d_to_e1d(G.energy3d(uk))
+    assert abs(E.sum() - L3 / 4) < 1e-12 * L3
+    D = G.dissipation_spectrum(uk, nu)
+    assert abs(D[k0] - nu * k0 ** 2 * L3 / 2) < 1e-12 * L3
+    assert abs(D.sum() - D[k0]) < 1e-14 * L3
