@@ -185,6 +185,14 @@ int ld_prhs(ld_ctx* ctx, const double* coord, int64_t np, const void* u, const v
  * spectral fields into out[N/2+1] (device double): weight 0.5 on u_hat gives
  * E(k) (forced-dns-sm.py:423, 431); weight 1 on n_hat the n spectrum (plot.py:37-38). */
 int ld_spectrum(ld_ctx* ctx, const void* f, int32_t nfields, double weight, double* out);
+/* ld_spectrum without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  No counterpart
+ * in the reference.  Same arguments and results (bit for bit) as ld_spectrum;
+ * its per-block partial sums live in a fixed 526,336-byte device array of the
+ * library (256 blocks x N/2+1 shells, N <= 512), one per GPU, looked up at
+ * ld_create.  Calls on one GPU share that array: calls that overlap in time
+ * on one GPU race and must be serialised by the caller. */
+int ld_spectrum_noalloc(ld_ctx* ctx, const void* f, int32_t nfields, double weight, double* out);
 /* Energy flux Pi(k): e3d_to_e1d(Re(conj(uk) . ku) * dealias * normalize),
  * cumulated from the highest shell down (forced-dns-sm.py:426-428); ku from
  * ld_full_rhs with visc = 0, forc = 0 (:425).  out[N/2+1] device double. */

@@ -94,6 +94,7 @@ _SIGS = {
     "ld_exterp_cosine_scalar": [_VP, _VP, ctypes.c_int64, _VP, _VP],
     "ld_prhs": [_VP, _VP, ctypes.c_int64, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
     "ld_spectrum": [_VP, _VP, ctypes.c_int32, ctypes.c_double, _VP],
+    "ld_spectrum_noalloc": [_VP, _VP, ctypes.c_int32, ctypes.c_double, _VP],
     "ld_flux": [_VP, _VP, _VP, _VP],
     "ld_field_stats": [_VP, _VP, ctypes.c_int32, _VP],
     "ld_dissipation": [_VP, _VP, _VP],

@@ -3,12 +3,25 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 struct ld_ctx {
   ld::EngineBase* eng = nullptr;
   ld_params p{};
+  int device = 0;     // the HIP device current at ld_create
 };
+
+// Scratch of ld_spectrum_noalloc: the per-block shell partial sums of the
+// spectrum (256 blocks x at most 257 shells, N <= 512: 526,336 bytes).  Each
+// GPU holds its own copy for the life of the process; its device address is
+// looked up once per device at ld_create and kept in g_scratch[].  Every call
+// on one GPU shares that GPU's copy, so such calls must not overlap.
+static constexpr size_t LD_SCRATCH_DOUBLES = 256 * 257;
+static __device__ __align__(256) double scratch[LD_SCRATCH_DOUBLES];
+static constexpr int LD_MAX_DEVICES = 64;
+static double* g_scratch[LD_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
 
 namespace ld {
 static thread_local char g_err[1024] = "";
@@ -21,6 +34,27 @@ void set_error(const char* fmt, ...) {
 }  // namespace ld
 
 using ld::set_error;
+
+// Device address of the scratch array on the current device, looked up once
+// per device (the array's host-side name is not a device address).
+static int scratch_setup(int* device_out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) { set_error("hipGetDevice failed: %s", hipGetErrorString(e)); return LD_ERR_HIP; }
+  if (dev < 0 || dev >= LD_MAX_DEVICES) { set_error("device %d beyond the %d supported", dev, LD_MAX_DEVICES); return LD_ERR_ARG; }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return LD_ERR_HIP;
+    }
+    g_scratch[dev] = static_cast<double*>(ptr);
+  }
+  *device_out = dev;
+  return LD_OK;
+}
 
 #define LD_CHECK_CTX(c)                                 \
   do {                                                  \
@@ -94,7 +128,13 @@ int ld_create(const ld_params* params, ld_ctx** out) {
     set_error("unsupported grid size N=%d (power of two in [16, 512])", params->N);
     return LD_ERR_ARG;
   }
-  int rc = e->init();
+  int dev = 0;
+  int rc = scratch_setup(&dev);
+  if (rc != LD_OK) {
+    delete e;
+    return rc;
+  }
+  rc = e->init();
   if (rc != LD_OK) {
     delete e;
     return rc;
@@ -107,6 +147,7 @@ int ld_create(const ld_params* params, ld_ctx** out) {
   }
   c->eng = e;
   c->p = *params;
+  c->device = dev;
   *out = c;
   return LD_OK;
 }
@@ -231,6 +272,14 @@ int ld_spectrum(ld_ctx* ctx, const void* f, int32_t nfields, double weight, doub
   LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(f); LD_NONNULL(out);
   if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
   return ctx->eng->spectrum(f, nfields, weight, out);
+}
+
+int ld_spectrum_noalloc(ld_ctx* ctx, const void* f, int32_t nfields, double weight, double* out) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(f); LD_NONNULL(out);
+  if (nfields < 1) { set_error("nfields must be >= 1"); return LD_ERR_ARG; }
+  double* ws = g_scratch[ctx->device];
+  if (!ws) { set_error("ld_spectrum_noalloc: no scratch address for device %d", ctx->device); return LD_ERR_HIP; }
+  return ctx->eng->spectrum_ws(f, nfields, weight, out, ws, LD_SCRATCH_DOUBLES);
 }
 
 int ld_flux(ld_ctx* ctx, const void* uk, const void* ku, double* out) {

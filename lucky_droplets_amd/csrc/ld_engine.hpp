@@ -1066,11 +1066,20 @@ struct Engine final : EngineBase {
   double norm0() const { return 1.0 / ((double)N * N * N * N * N * N / std::pow(2.0 * M_PI, 3.0)); }
   int spectrum(const void* f, int nf, double w, double* out) override {
     LD_TRY(diag_alloc());
+    return spectrum_ws(f, nf, w, out, dpart, (size_t)NBLK_DIAG * (Nf > 4 ? Nf : 4));
+  }
+  // The shell partial sums go to ws (every entry the final sum reads is
+  // written first, so ws needs no clearing); nothing is allocated.
+  int spectrum_ws(const void* f, int nf, double w, double* out, double* ws, size_t ws_len) override {
+    if ((size_t)NBLK_DIAG * Nf > ws_len) {
+      set_error("spectrum scratch of %zu doubles is below the %zu this grid needs", ws_len, (size_t)NBLK_DIAG * Nf);
+      return LD_ERR_ARG;
+    }
     LD_TRY(join(st, st2, evB));
     hipLaunchKernelGGL((k_shell_partial<T, N>), dim3(NBLK_DIAG), dim3(256), 0, st, (const C<T>*)f, (const C<T>*)nullptr,
-                       nf, CF, 0, w, norm0(), dpart);
+                       nf, CF, 0, w, norm0(), ws);
     LD_LAUNCHED();
-    hipLaunchKernelGGL((k_shell_final<N>), dim3(1), dim3(256), 0, st, (const double*)dpart, NBLK_DIAG, 0, out);
+    hipLaunchKernelGGL((k_shell_final<N>), dim3(1), dim3(256), 0, st, (const double*)ws, NBLK_DIAG, 0, out);
     LD_LAUNCHED();
     return LD_OK;
   }

@@ -37,6 +37,8 @@ struct EngineBase {
   virtual int coalesce(double* rows, int64_t np, const int64_t* pairs, int64_t npairs, int64_t* events,
                        int64_t* nevents) = 0;
   virtual int spectrum(const void* f, int nf, double w, double* out) = 0;
+  // spectrum with caller-provided scratch ws (ws_len doubles); allocates nothing
+  virtual int spectrum_ws(const void* f, int nf, double w, double* out, double* ws, size_t ws_len) = 0;
   virtual int flux(const void* uk, const void* ku, double* out) = 0;
   virtual int dissipation(const void* uk, double* out) = 0;
   virtual int divergence(const void* uk, void* div) = 0;
