@@ -77,9 +77,6 @@ def build(jobs: int | None = None, force: bool = True, verbose: bool = False) ->
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
     os.replace(tmp, LIB)
-    # the traffic-counter calibration program (a 1 GiB copy; see profiles/)
-    cal = HERE / "copy_calibrate"
-    _run([HIPCC, *[f for f in BASE_FLAGS if f != "-fPIC"], str(CSRC / "copy_calibrate.hip"), "-o", str(cal)])
     return LIB
 
 
