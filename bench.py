@@ -233,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64 x 64^3 ensemble timing")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-against-FP64 accuracy run")
     ap.add_argument("--no-cnn", action="store_true", help="skip the learned-subgrid CNN timing")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default y-pass block shape")
     args = ap.parse_args()
@@ -316,6 +317,16 @@ def main():
         out["fp64_plain"] = {"ms_per_step": msp, "steps_per_s": 1000.0 / msp, "fp32_speedup": msp / ms32,
                              "form": "reference dataflow on the GPU: 74 separate 3-D transforms per step and one "
                                      "kernel per array statement (ld_advance_plain)"}
+    if not args.no_fp64 and not args.no_accuracy:
+        from lucky_droplets_amd.accuracy import compare_precisions
+        nacc = cfg.steps
+        log(f"[bench] FP32 against FP64 over {nacc} steps")
+        rep = compare_precisions(cfg, uk, n, coord, dt, nacc)
+        rep.pop("ek_f32")
+        rep.pop("ek_f64")
+        out["fp32_accuracy"] = rep
+        log(f"[bench] spectrum error max {rep['spectrum_err_max']:.2e} (shell {rep['spectrum_err_max_shell']}), "
+            f"dissipation error {rep['dissipation_err']:.2e}")
     if not args.no_ensemble:
         from lucky_droplets_amd.config import preset as _preset
         c3 = _preset("c3")

@@ -98,3 +98,27 @@ def box_filter(u: np.ndarray) -> np.ndarray:
             for c in (-1, 0, 1):
                 out += w[a + 1] * w[b + 1] * w[c + 1] * shift(u, (a, b, c))
     return out[::2, ::2, ::2]
+
+
+def forward_at(layers, x, nodes, bf16=True):
+    """The network output at the listed nodes only (nodes (S, 3) int), with the
+    rounding points of ``forward``: each output depends on the (2L+1)^3 input
+    neighbourhood of its node, which is cut out periodically and convolved
+    without padding, layer by layer.  Equals ``forward(layers, x)[0][nodes]``."""
+    rd = to_bf16 if bf16 else (lambda v: np.asarray(v, dtype=np.float64))
+    L = len(layers)
+    N = x.shape[0]
+    r = np.arange(-L, L + 1)
+    out = []
+    for n in np.asarray(nodes):
+        ix = [(n[d] + r) % N for d in range(3)]
+        a = rd(x[np.ix_(ix[0], ix[1], ix[2])])
+        for l, (W, b) in enumerate(layers):
+            m = a.shape[0] - 2
+            Wr = rd(W)
+            z = np.zeros((m, m, m, W.shape[1])) + np.asarray(b, dtype=np.float64)
+            for t, off in enumerate(OFFSETS):
+                z += a[1 + off[0]: 1 + off[0] + m, 1 + off[1]: 1 + off[1] + m, 1 + off[2]: 1 + off[2] + m] @ Wr[t].T
+            a = rd(np.maximum(z, 0.0)) if l + 1 < L else z
+        out.append(a[0, 0, 0])
+    return np.array(out)

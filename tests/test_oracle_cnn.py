@@ -67,3 +67,17 @@ def test_box_filter_of_constant_and_linear_mode():
     got = cnn.box_filter(v)
     h = 0.5 + 0.5 * np.cos(k * 2 * np.pi / 8)
     assert np.allclose(got, h * v[::2, ::2, ::2], atol=1e-12)
+
+
+def test_forward_at_equals_full_forward():
+    """The node-subset forward (used at grids where the full CPU forward is too
+    slow) equals the full periodic forward at those nodes, including nodes
+    whose neighbourhood wraps around the box."""
+    rng = np.random.default_rng(3)
+    N, ch = 12, (3, 8, 8, 2)
+    layers = [(rng.normal(0, 0.3, (27, co, ci)), rng.normal(0, 0.1, co)) for ci, co in zip(ch[:-1], ch[1:])]
+    x = rng.normal(size=(N, N, N, ch[0]))
+    y, _ = cnn.forward(layers, x, bf16=True)
+    nodes = np.array([[0, 0, 0], [N - 1, 5, 0], [3, 7, 11], [6, 6, 6], [11, 11, 11]])
+    ya = cnn.forward_at(layers, x, nodes, bf16=True)
+    assert np.max(np.abs(ya - y[nodes[:, 0], nodes[:, 1], nodes[:, 2]])) < 1e-12
