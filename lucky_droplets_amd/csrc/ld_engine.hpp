@@ -125,6 +125,7 @@ struct Engine final : EngineBase {
   PT* v0b = nullptr;
   double4* dep = nullptr;            // per-slot stage position and W for the brick deposit
   int *flist = nullptr, *fcount = nullptr;
+  unsigned long long* fx = nullptr;  // fixed-point scratch field of the fallback deposit (zero between steps)
   unsigned long long* wmaxp = nullptr;   // largest W of a droplet stage (fixed-point deposit bound)
   bool sort_enabled = true;
   // y and z passes of a chain run slab by slab (x-planes) so the y<->z
@@ -285,7 +286,7 @@ struct Engine final : EngineBase {
       arena(rid, cap); arena(rid2, cap); arena(pkey, cap); arena(prank, cap);
       arena(hist, NB); arena(bsum, NBS);
       arena(x0b, 3 * cap); arena(v0b, 4 * cap);
-      arena(dep, cap); arena(flist, cap); arena(fcount, 1); arena(wmaxp, 1);
+      arena(dep, cap); arena(flist, cap); arena(fcount, 1); arena(wmaxp, 1); arena(fx, RF);
     }
     LD_TRY(arena_commit());
     LD_TRY(alloc(shells, 8));
@@ -300,6 +301,10 @@ struct Engine final : EngineBase {
       LD_HIP(hipMemcpy(at(clip_none, b), cn, sizeof(cn), hipMemcpyHostToDevice));
       LD_HIP(hipMemcpy(at(clip_stage, b), cn, sizeof(cn), hipMemcpyHostToDevice));
       LD_HIP(hipMemcpy(at(clip_end, b), cn, sizeof(cn), hipMemcpyHostToDevice));
+    }
+    if (cap > 0) {
+      LD_TRY(fill<unsigned long long>(st2, fx, RF, 0ull));
+      LD_HIP(hipStreamSynchronize(st2));
     }
     static_assert(NBS <= 4096, "bin-start scan supports at most 4096 partial sums");
     dp.L = p.L;
@@ -597,8 +602,11 @@ struct Engine final : EngineBase {
         LD_LAUNCHED();
       }
       const unsigned fb = sorted_ok ? 64u : (unsigned)((np + 255) / 256);
-      hipLaunchKernelGGL((k_fallback_deposit<T, N>), dim3(fb, 1, B), dim3(256), 0, st2, (const double4*)dep,
-                         (const int*)flist, (const int*)fcount, fc, dp, rs);
+      hipLaunchKernelGGL((k_fallback_accum<T, N>), dim3(fb, 1, B), dim3(256), 0, st2, (const double4*)dep,
+                         (const int*)flist, (const int*)fcount, (const unsigned long long*)wmaxp, fx, dp, rs);
+      LD_LAUNCHED();
+      hipLaunchKernelGGL((k_fallback_fold<T, N>), dim3(fb, 1, B), dim3(256), 0, st2, (const double4*)dep,
+                         (const int*)flist, (const int*)fcount, (const unsigned long long*)wmaxp, fx, fc, dp, rs);
       LD_LAUNCHED();
     }
     OpZKn<T, N, P1> ok;

@@ -405,10 +405,12 @@ k_particles_stage(PStage<T, PT> A0) {
   m = (double)(PT)(v0v[3] + (PT)ah * kvv[3]);
   double rhs[7], W = 0.0;
   if (live) dropletRhs(A.P, vp, m, I, rhs, W);
-  // largest growth rate of the launch: the bound of the brick deposit's fixed-point scale
+  // largest growth rate of the launch: the bound of the deposits' fixed-point
+  // scale.  A NaN or negative W (not a growth) makes it infinite, so the
+  // deposit writes NaN everywhere, as the reference's would spread it.
   {
     __shared__ double wm[4];
-    double v = (live && W > 0.0) ? W : 0.0;
+    double v = !live ? 0.0 : (W > 0.0 ? W : (W == 0.0 ? 0.0 : INFINITY));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
@@ -419,8 +421,8 @@ k_particles_stage(PStage<T, PT> A0) {
     }
   }
   if (!live || c != 0) return;
-  // a fallback slot stores -(W + 1) so the brick pass skips it
-  reinterpret_cast<double*>(A.dep + p)[3] = far ? -(W + 1.0) : W;
+  // a fallback slot stores -W so the brick pass (which takes W > 0) skips it
+  reinterpret_cast<double*>(A.dep + p)[3] = far ? -W : W;
   // sump (forced-dns-sm.py:337, 342-358); the last stage writes the new state
   double* dxp = A.stage == 3 ? A.x0 : A.sx;
   PT* dvp = A.stage == 3 ? A.v0 : A.sv;
@@ -623,23 +625,94 @@ k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0
   }
 }
 
-// Fallback deposit (global atomics) of the slots that moved more than one
-// bin since the sort; runs after k_brick_deposit.
+// Fallback deposit of the slots that moved more than one bin since the sort
+// (or of every slot when the slots are not sorted); runs after
+// k_brick_deposit.  Deterministic like the bricks: k_fallback_accum adds each
+// slot's 64 node values as 64-bit fixed-point integers into the scratch field
+// fx (zero between steps; integer sums do not depend on the order of the
+// additions), with the scale 2^e of the largest value any node can reach
+// (slots x Wmax / dx^3 / 8); k_fallback_fold then moves each touched node's
+// sum into fc and clears it (the first slot to exchange a node takes its whole
+// sum; the others find zero).
+struct FallbackScale {
+  double scale, unit;
+  bool finite;
+  __device__ FallbackScale(int nf, unsigned long long wmax_bits, double inv_vol) {
+    const double bound = (double)nf * __longlong_as_double((long long)wmax_bits) * inv_vol * 0.125;
+    finite = isfinite(bound);
+    const int e = (bound > 0.0 && finite) ? 61 - ilogb(bound) : 0;
+    scale = ldexp(1.0, e);
+    unit = ldexp(1.0, -e);
+  }
+};
+
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
-k_fallback_deposit(const double4* __restrict__ dep0, const int* __restrict__ flist0,
-                   const int* __restrict__ fcount0, T* fc0, DropPhys P, size_t rs) {
+k_fallback_accum(const double4* __restrict__ dep0, const int* __restrict__ flist0, const int* __restrict__ fcount0,
+                 const unsigned long long* __restrict__ wmax0, unsigned long long* fx0, DropPhys P, size_t rs) {
   const double4* __restrict__ dep = rsel1(dep0, rs);
   const int* __restrict__ flist = rsel1(flist0, rs);
-  const int* __restrict__ fcount = rsel1(fcount0, rs);
-  T* fc = rsel1(fc0, rs);
-  const int nf = *fcount;
+  unsigned long long* fx = rsel1(fx0, rs);
+  const int nf = *rsel1(fcount0, rs);
+  if (nf == 0) return;
+  const FallbackScale sc(nf, *rsel1(wmax0, rs), P.inv_vol);
+  if (!sc.finite) return;
+  using GT = std::conditional_t<sizeof(T) == 4, float, double>;
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
     const double4 d = dep[flist[f]];
-    const double xw[3] = {d.x, d.y, d.z};
-    Stencil<N> s;
-    s.set(xw, P.dx, P.pi_2dx);
-    deposit<T, N>(s, -d.w - 1.0, P.inv_vol, fc);
+    const double w = -d.w;                       // fallback slots carry -W
+    if (!(w > 0.0)) continue;
+    GT wx[4], wy[4], wz[4];
+    const int i0 = cosW4G<GT>(d.x, P.dx, P.pi_2dx, wx), j0 = cosW4G<GT>(d.y, P.dx, P.pi_2dx, wy),
+              k0 = cosW4G<GT>(d.z, P.dx, P.pi_2dx, wz);
+    const double W = w * P.inv_vol * sc.scale;
+#pragma unroll 1
+    for (int a = 0; a < 4; ++a) {
+      const double wa = W * (double)wx[a];
+      const size_t plane = (size_t)wrapIdx<N>(i0 + a) * N;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const double wab = wa * (double)wy[b];
+        const size_t row = (plane + wrapIdx<N>(j0 + b)) * N;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          atomicAdd(&fx[row + wrapIdx<N>(k0 + c)], (unsigned long long)llrint(wab * (double)wz[c]));
+      }
+    }
+  }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_fallback_fold(const double4* __restrict__ dep0, const int* __restrict__ flist0, const int* __restrict__ fcount0,
+                const unsigned long long* __restrict__ wmax0, unsigned long long* fx0, T* fc0, DropPhys P, size_t rs) {
+  const double4* __restrict__ dep = rsel1(dep0, rs);
+  const int* __restrict__ flist = rsel1(flist0, rs);
+  unsigned long long* fx = rsel1(fx0, rs);
+  T* fc = rsel1(fc0, rs);
+  const int nf = *rsel1(fcount0, rs);
+  if (nf == 0) return;
+  const FallbackScale sc(nf, *rsel1(wmax0, rs), P.inv_vol);
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+    const double4 d = dep[flist[f]];
+    const int i0 = (int)floor(d.x / P.dx) - 1, j0 = (int)floor(d.y / P.dx) - 1, k0 = (int)floor(d.z / P.dx) - 1;
+    // a NaN or negative growth somewhere: NaN at every node the slot reaches
+    const bool poison = !sc.finite;
+#pragma unroll 1
+    for (int a = 0; a < 4; ++a) {
+      const size_t plane = (size_t)wrapIdx<N>(i0 + a) * N;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const size_t row = (plane + wrapIdx<N>(j0 + b)) * N;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const size_t o = row + wrapIdx<N>(k0 + c);
+          if (poison) { fc[o] = (T)NAN; continue; }
+          const unsigned long long v = atomicExch(&fx[o], 0ull);
+          if (v) fc[o] = fc[o] + (T)((double)v * sc.unit);
+        }
+      }
+    }
   }
 }
 
