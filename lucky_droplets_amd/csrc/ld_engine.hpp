@@ -145,6 +145,7 @@ struct Engine final : EngineBase {
   ~Engine() override {
     if (st2) { (void)hipStreamSynchronize(st2); (void)hipStreamDestroy(st2); }
     if (stg) { (void)hipStreamSynchronize(stg); (void)hipStreamDestroy(stg); }
+    if (stf) { (void)hipStreamSynchronize(stf); (void)hipStreamDestroy(stf); }
     for (StepGraph& g : graphs)
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
@@ -261,10 +262,29 @@ struct Engine final : EngineBase {
     }
     LD_TRY(alloc(tw, N));
     LD_HIP(hipMemcpy(tw, h.data(), N * sizeof(C<T>), hipMemcpyHostToDevice));
-    LD_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));   // default priority
+    // LD_CU_MASK=<8 hex words>: the density/droplet stream runs on the CUs of the
+    // mask and the step's flow stream (internal) on the others
+    std::vector<uint32_t> dmask;
+    if (const char* e = getenv("LD_CU_MASK")) {
+      const char* q = e;
+      while (*q && dmask.size() < 16) {
+        char* end = nullptr;
+        dmask.push_back((uint32_t)strtoul(q, &end, 16));
+        if (end == q) break;
+        q = (*end == ',') ? end + 1 : end;
+      }
+    }
+    if (!dmask.empty()) {
+      std::vector<uint32_t> fmask(dmask.size());
+      for (size_t i = 0; i < dmask.size(); ++i) fmask[i] = ~dmask[i];
+      LD_HIP(hipExtStreamCreateWithCUMask(&st2, (uint32_t)dmask.size() * 32, dmask.data()));
+      LD_HIP(hipExtStreamCreateWithCUMask(&stf, (uint32_t)fmask.size() * 32, fmask.data()));
+    } else {
+      LD_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));   // default priority
+    }
     use_graph = p.use_graph != 0;
-    if (use_graph) {
-      LD_HIP(hipStreamCreateWithFlags(&stg, hipStreamNonBlocking));
+    if (use_graph) LD_HIP(hipStreamCreateWithFlags(&stg, hipStreamNonBlocking));
+    if (use_graph || stf) {
       LD_HIP(hipEventCreateWithFlags(&evG0, hipEventDisableTiming));
       LD_HIP(hipEventCreateWithFlags(&evG1, hipEventDisableTiming));
     }
@@ -312,6 +332,9 @@ struct Engine final : EngineBase {
     dp.pi_2dx = M_PI / (2.0 * dp.dx);
     dp.inv_vol = 1.0 / (dp.dx * dp.dx * dp.dx);
     dp.factor = p.factor; dp.growth = p.growthfactor; dp.st_s = p.st_s; dp.tau_eta = p.tau_eta; dp.g = p.g;
+    dp.invL = 1.0 / p.L;
+    dp.inv_dx = (double)N / p.L;
+    dp.inv_factor = 1.0 / p.factor;
     if (p.autotune) LD_TRY(autotune());
     return LD_OK;
   }
@@ -746,7 +769,7 @@ struct Engine final : EngineBase {
     const unsigned pb = (unsigned)((np + 255) / 256);
     LD_TRY(fill<int>(s, hist, NB, 0));
     hipLaunchKernelGGL((k_bin_count<N>), dim3(pb, 1, B), dim3(256), 0, s, (const double*)x0, (int)np, (int)cap, p.L,
-                       dp.dx, pkey, prank, hist, rs);
+                       dp.invL, dp.inv_dx, pkey, prank, hist, rs);
     LD_LAUNCHED();
     hipLaunchKernelGGL(k_scan_local, dim3(NBS, 1, B), dim3(1024), 0, s, hist, NB, bsum, rs);
     LD_LAUNCHED();
@@ -780,6 +803,7 @@ struct Engine final : EngineBase {
   struct StepGraph { hipGraphExec_t exec = nullptr; double h = 0.0; int64_t np = -1; bool band = false; };
   bool use_graph = false;
   hipStream_t stg = nullptr;
+  hipStream_t stf = nullptr;          // flow stream with a CU mask (LD_CU_MASK), else unused
   hipEvent_t evG0 = nullptr, evG1 = nullptr;   // caller <-> internal stream joins
   StepGraph graphs[2];
   int parity = 0;                    // droplet buffers swapped an odd number of times
@@ -830,6 +854,16 @@ struct Engine final : EngineBase {
       st = caller;
       LD_TRY(rc);
       LD_TRY(join(caller, stg, evG1));
+    } else if (stf) {
+      hipStream_t caller = st;
+      LD_TRY(join(stf, caller, evG0));
+      st = stf;
+      int rc = join(st2, st, evA);
+      for (int i = 0; i < nsteps && rc == LD_OK; ++i) rc = step_body(h);
+      if (rc == LD_OK) rc = join(st, st2, evB);
+      st = caller;
+      LD_TRY(rc);
+      LD_TRY(join(caller, stf, evG1));
     } else {
       LD_TRY(join(st2, st, evA));      // state written on the caller's stream
       for (int i = 0; i < nsteps; ++i) LD_TRY(step_body(h));

@@ -73,6 +73,42 @@ __device__ __forceinline__ double wrapL(double x, double L) {
   return r;
 }
 
+// The step's droplet kernels (stage gather, sort bins, brick and fallback
+// deposits) wrap and bin positions with multiplications by 1/L and 1/dx
+// instead of fmod and divisions; all of them use these same three functions,
+// so their cells agree.  The wrapped position can differ from fmod's in the
+// last bit, and a stencil origin can differ by one cell when x/dx is within
+// an ulp of an integer; the cosine kernel gives the node at distance 2 dx
+// zero weight, so both stencils give the same interpolant.
+__device__ __forceinline__ double wrapFast(double x, double L, double invL) {
+  double r = fma(-L, floor(x * invL), x);
+  r = r < 0.0 ? r + L : r;
+  r = r >= L ? r - L : r;
+  if (!(r >= 0.0 && r < L)) r = 0.0;   // a non-finite position must not become an out-of-range index
+  return r;
+}
+// cell of a wrapped position, clamped to [0, N)
+template <int N> __device__ __forceinline__ int cellOf(double xw, double inv_dx) {
+  const int c = (int)(xw * inv_dx);
+  return c < 0 ? 0 : (c > N - 1 ? N - 1 : c);
+}
+// stencil origin i0 = floor(x / dx) - 1 (nodes i0 .. i0 + 3)
+__device__ __forceinline__ int originOf(double x, double inv_dx) { return (int)floor(x * inv_dx) - 1; }
+
+// cosW4G with the origin from originOf.
+template <typename G>
+__device__ __forceinline__ int cosW4F(double x, double dx, double inv_dx, double pi_2dx, G w[4]) {
+  const int i0 = originOf(x, inv_dx);
+  G sn, cs;
+  sincosGT<G>((x - (double)i0 * dx) * pi_2dx, sn, cs);
+  const G q = (G)0.25;
+  w[0] = q * ((G)1 + cs);
+  w[1] = q * ((G)1 + sn);
+  w[2] = q * ((G)1 - cs);
+  w[3] = q * ((G)1 - sn);
+  return i0;
+}
+
 // Geometry of one droplet's stencil.
 template <int N> struct Stencil {
   int i0;
@@ -165,6 +201,7 @@ __device__ __forceinline__ void deposit(const Stencil<N>& s, double W, double in
 struct DropPhys {
   double L, dx, pi_2dx, inv_vol;
   double factor, growth, st_s, tau_eta, g;
+  double invL, inv_dx, inv_factor;
 };
 
 // Droplet RHS at one stage coordinate (particles.py:823-835).  xw: wrapped
@@ -182,6 +219,24 @@ __device__ __forceinline__ void dropletRhs(const DropPhys& P, const double vp[3]
   rhs[3] = (I[0] - vp[0]) / den;                                            // :832
   rhs[4] = (I[1] - vp[1]) / den;
   rhs[5] = (I[2] - vp[2]) / den - P.g;                                      // :834
+  rhs[6] = W;                                                               // :835
+}
+
+// The same RHS in single precision for the FP32 form (velocities and masses
+// are FP32 there); St from exp2(2/3 log2(m / factor)), divisions by one
+// reciprocal.
+__device__ __forceinline__ void dropletRhsF(const DropPhys& P, const float vp[3], float m, const float I[7],
+                                            float rhs[7], float& W) {
+  const float st = exp2f((2.0f / 3.0f) * log2f(m * (float)P.inv_factor));   // :823
+  const float d0 = vp[0] - I[3], d1 = vp[1] - I[4], d2 = vp[2] - I[5];
+  const float nrm = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+  const float q = 1.0f + sqrtf((float)P.st_s / st);
+  W = (float)P.growth * (q * q) * I[6] * nrm;                                // :828
+  rhs[0] = vp[0]; rhs[1] = vp[1]; rhs[2] = vp[2];                           // :831
+  const float rden = 1.0f / (st * (float)P.tau_eta);
+  rhs[3] = (I[0] - vp[0]) * rden;                                           // :832
+  rhs[4] = (I[1] - vp[1]) * rden;
+  rhs[5] = (I[2] - vp[2]) * rden - (float)P.g;                              // :834
   rhs[6] = W;                                                               // :835
 }
 
@@ -280,8 +335,8 @@ k_particles_stage(PStage<T, PT> A0) {
   for (int k = 0; k < 3; ++k) x0v[k] = A.x0[k * cap + pp];
 #pragma unroll
   for (int k = 0; k < 3; ++k) xs[k] = x0v[k] + ah * kxp[k * cap + pp];
-  const double L = A.P.L, dx = A.P.dx, pi_2dx = A.P.pi_2dx;
-  const double xw = wrapL(xs[0], L), yw = wrapL(xs[1], L), zw = wrapL(xs[2], L);
+  const double L = A.P.L, dx = A.P.dx, pi_2dx = A.P.pi_2dx, invL = A.P.invL, inv_dx = A.P.inv_dx;
+  const double xw = wrapFast(xs[0], L, invL), yw = wrapFast(xs[1], L, invL), zw = wrapFast(xs[2], L, invL);
   // one sincos per axis (cosW4); node i0 + a has weight (1 +- {cos, sin})/4.
   // The angles and the sincos are FP64; the weights and the gather sums are
   // kept in the field precision GT (FP32 fields: FP32 sums of 64 terms).
@@ -297,9 +352,7 @@ k_particles_stage(PStage<T, PT> A0) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       // bins exactly as k_bin_count forms them
-      int cs = (int)(xb[k] / dx), c0 = (int)(wrapL(x0v[k], L) / dx);
-      cs = cs < 0 ? 0 : (cs > N - 1 ? N - 1 : cs);
-      c0 = c0 < 0 ? 0 : (c0 > N - 1 ? N - 1 : c0);
+      const int cs = cellOf<N>(xb[k], inv_dx), c0 = cellOf<N>(wrapFast(x0v[k], L, invL), inv_dx);
       const int d = ((cs / Bins<N>::BS - c0 / Bins<N>::BS) % Bins<N>::B + Bins<N>::B) % Bins<N>::B;
       far = far || (d > 1 && d < Bins<N>::B - 1);
     }
@@ -307,7 +360,7 @@ k_particles_stage(PStage<T, PT> A0) {
     dp[0] = xw; dp[1] = yw; dp[2] = zw;
     if (far) A.flist[atomicAdd(A.fcount, 1)] = p;
   }
-  const int i0 = (int)floor(xw / dx) - 1, j0 = (int)floor(yw / dx) - 1, k0 = (int)floor(zw / dx) - 1;
+  const int i0 = originOf(xw, inv_dx), j0 = originOf(yw, inv_dx), k0 = originOf(zw, inv_dx);
   // the angles are formed in FP64; their sines and cosines are taken in the
   // weight precision GT (the FP32 form rounds the angle in [pi/2, pi) to FP32)
   GT xsn, xcs, ysn, ycs, zsn, zcs;
@@ -399,12 +452,25 @@ k_particles_stage(PStage<T, PT> A0) {
   for (int k = 0; k < 4; ++k) { v0v[k] = A.v0[k * cap + pp]; kvv[k] = kvp[k * cap + pp]; prv[k] = svp[k * cap + pp]; }
 #pragma unroll
   for (int k = 0; k < 3; ++k) prx[k] = sxp[k * cap + pp];
-  double vp[3], m;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) vp[k] = (double)(PT)(v0v[k] + (PT)ah * kvv[k]);
-  m = (double)(PT)(v0v[3] + (PT)ah * kvv[3]);
   double rhs[7], W = 0.0;
-  if (live) dropletRhs(A.P, vp, m, I, rhs, W);
+  if constexpr (sizeof(PT) == 4) {
+    float vpf[3], If[7], rf[7], Wf = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vpf[k] = v0v[k] + (PT)ah * kvv[k];
+    const float mf = v0v[3] + (PT)ah * kvv[3];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) If[k] = (float)I[k];
+    if (live) dropletRhsF(A.P, vpf, mf, If, rf, Wf);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) rhs[k] = (double)rf[k];
+    W = (double)Wf;
+  } else {
+    double vp[3], m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vp[k] = (double)(PT)(v0v[k] + (PT)ah * kvv[k]);
+    m = (double)(PT)(v0v[3] + (PT)ah * kvv[3]);
+    if (live) dropletRhs(A.P, vp, m, I, rhs, W);
+  }
   // largest growth rate of the launch: the bound of the deposits' fixed-point
   // scale.  A NaN or negative W (not a growth) makes it infinite, so the
   // deposit writes NaN everywhere, as the reference's would spread it.
@@ -446,9 +512,9 @@ k_particles_stage(PStage<T, PT> A0) {
 // deposit is bitwise reproducible, and ds_add_u64 is the fastest LDS atomic
 // wide enough (gfx950: u64 15, f64 27, f32 196 cycles per wavefront).  The
 // scale 2^e is the largest power of two with  total * Wmax / dx^3 / 8 * 2^e
-// <= 2^62  (total = candidates of the brick, Wmax = largest W of the launch,
-// 1/8 = largest product of three weights), so no sum can overflow; one unit
-// is ~2^-62 of that bound.
+// < 2^62, 2^52 for FP32 fields  (total = candidates of the brick, Wmax =
+// largest W of the launch, 1/8 = largest product of three weights), so no sum
+// can overflow; one unit is ~2^-62 (2^-52) of that bound.
 //
 // The droplets that can reach the brick are those whose stage cell lies in
 // [c0 - 2, c0 + S] per axis; their sorted bins (bins of the position at the
@@ -547,10 +613,15 @@ k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0
   }
   __syncthreads();
   const int total = rl[NR];
-  const double dx = P.dx, pi_2dx = P.pi_2dx;
+  const double dx = P.dx, pi_2dx = P.pi_2dx, inv_dx = P.inv_dx;
   const double bound = (double)total * __longlong_as_double((long long)*wmax) * P.inv_vol * 0.125;
   const bool finite = isfinite(bound);
-  const int e = (bound > 0.0 && finite) ? 61 - ilogb(bound) : 0;
+  // FP32 fields: sums below 2^52, so a value converts to its integer by one
+  // addition of 2^52 (round to nearest even, as llrint); FP64 fields keep the
+  // 2^62 range and llrint
+  constexpr bool F32 = sizeof(T) == 4;
+  constexpr int EMAX = F32 ? 51 : 61;
+  const int e = (bound > 0.0 && finite) ? EMAX - ilogb(bound) : 0;
   const double scale = ldexp(1.0, e), unit = ldexp(1.0, -e);
   for (int base = 0; base < total && finite; base += CH) {
     // test and compact: growing candidates whose stencil meets the brick
@@ -568,9 +639,9 @@ k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0
         t = rb[a] + (f - rl[a]);
         const double4 d = dep[t];
         if (d.w > 0.0) {
-          const int rx = (((int)floor(d.x / dx) - 1 - c0x) % N + N) % N;
-          const int ry = (((int)floor(d.y / dx) - 1 - c0y) % N + N) % N;
-          const int rz = (((int)floor(d.z / dx) - 1 - c0z) % N + N) % N;
+          const int rx = ((originOf(d.x, inv_dx) - c0x) % N + N) % N;
+          const int ry = ((originOf(d.y, inv_dx) - c0y) % N + N) % N;
+          const int rz = ((originOf(d.z, inv_dx) - c0z) % N + N) % N;
           take = (rx < S || rx > N - 4) && (ry < S || ry > N - 4) && (rz < S || rz > N - 4);
         }
       }
@@ -589,8 +660,8 @@ k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0
       // stencil weights in the field precision (FP32 fields: FP32 sines and cosines)
       using GT = std::conditional_t<sizeof(T) == 4, float, double>;
       GT wx[4], wy[4], wz[4];
-      const int i0 = cosW4G<GT>(d.x, dx, pi_2dx, wx), j0 = cosW4G<GT>(d.y, dx, pi_2dx, wy),
-                k0 = cosW4G<GT>(d.z, dx, pi_2dx, wz);
+      const int i0 = cosW4F<GT>(d.x, dx, inv_dx, pi_2dx, wx), j0 = cosW4F<GT>(d.y, dx, inv_dx, pi_2dx, wy),
+                k0 = cosW4F<GT>(d.z, dx, inv_dx, pi_2dx, wz);
       const int rx = ((i0 - c0x) % N + N) % N, ry = ((j0 - c0y) % N + N) % N, rz = ((k0 - c0z) % N + N) % N;
       // scale is a power of two, so W * scale is exact and the products below
       // round as (W wx wy wz) scale would
@@ -610,7 +681,13 @@ k_brick_deposit(const double4* __restrict__ dep0, const int* __restrict__ start0
           for (int cc = 0; cc < 4; ++cc) {
             const int nz = (rz + cc) % N;
             if (nz >= S) continue;
-            atomicAdd(&tile[(nx * S + ny) * S + nz], (unsigned long long)llrint(wab * wzd[cc]));
+            const double v = wab * wzd[cc];
+            unsigned long long q;
+            if constexpr (F32)
+              q = (unsigned long long)(__double_as_longlong(v + 0x1p52) - __double_as_longlong(0x1p52));
+            else
+              q = (unsigned long long)llrint(v);
+            atomicAdd(&tile[(nx * S + ny) * S + nz], q);
           }
         }
       }
@@ -663,8 +740,8 @@ k_fallback_accum(const double4* __restrict__ dep0, const int* __restrict__ flist
     const double w = -d.w;                       // fallback slots carry -W
     if (!(w > 0.0)) continue;
     GT wx[4], wy[4], wz[4];
-    const int i0 = cosW4G<GT>(d.x, P.dx, P.pi_2dx, wx), j0 = cosW4G<GT>(d.y, P.dx, P.pi_2dx, wy),
-              k0 = cosW4G<GT>(d.z, P.dx, P.pi_2dx, wz);
+    const int i0 = cosW4F<GT>(d.x, P.dx, P.inv_dx, P.pi_2dx, wx), j0 = cosW4F<GT>(d.y, P.dx, P.inv_dx, P.pi_2dx, wy),
+              k0 = cosW4F<GT>(d.z, P.dx, P.inv_dx, P.pi_2dx, wz);
     const double W = w * P.inv_vol * sc.scale;
 #pragma unroll 1
     for (int a = 0; a < 4; ++a) {
@@ -695,7 +772,7 @@ k_fallback_fold(const double4* __restrict__ dep0, const int* __restrict__ flist0
   const FallbackScale sc(nf, *rsel1(wmax0, rs), P.inv_vol);
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
     const double4 d = dep[flist[f]];
-    const int i0 = (int)floor(d.x / P.dx) - 1, j0 = (int)floor(d.y / P.dx) - 1, k0 = (int)floor(d.z / P.dx) - 1;
+    const int i0 = originOf(d.x, P.inv_dx), j0 = originOf(d.y, P.inv_dx), k0 = originOf(d.z, P.inv_dx);
     // a NaN or negative growth somewhere: NaN at every node the slot reaches
     const bool poison = !sc.finite;
 #pragma unroll 1
@@ -803,7 +880,7 @@ static __global__ void k_axpy_rows(const double* __restrict__ x, const double* _
 
 template <int N>
 __global__ void __launch_bounds__(256)
-k_bin_count(const double* __restrict__ pos0, int np, int cap, double L, double dx, int* __restrict__ key0,
+k_bin_count(const double* __restrict__ pos0, int np, int cap, double L, double invL, double inv_dx, int* __restrict__ key0,
             int* __restrict__ rank0, int* __restrict__ hist0, size_t rs) {
   const double* __restrict__ pos = rsel1(pos0, rs);
   int* __restrict__ key = rsel1(key0, rs);
@@ -814,9 +891,7 @@ k_bin_count(const double* __restrict__ pos0, int np, int cap, double L, double d
   int b[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    int ci = (int)(wrapL(pos[c * cap + p], L) / dx);
-    ci = ci < 0 ? 0 : (ci > N - 1 ? N - 1 : ci);
-    b[c] = ci / Bins<N>::BS;
+    b[c] = cellOf<N>(wrapFast(pos[c * cap + p], L, invL), inv_dx) / Bins<N>::BS;
   }
   const int k = Bins<N>::key(b[0], b[1], b[2]);
   key[p] = k;
