@@ -99,8 +99,10 @@ struct Engine final : EngineBase {
   hipEvent_t evA = nullptr, evB = nullptr;
   C<T>*uk = nullptr, *us = nullptr, *acc = nullptr, *w6 = nullptr, *wn = nullptr;
   T* ubuf[2] = {nullptr, nullptr};       // planar physical u (API / plain paths)
-  V4<T>* gu[2] = {nullptr, nullptr};     // node-interleaved {u, 0}, one per stage parity
-  V4<T>* gv = nullptr;                   // node-interleaved {v, n_stage}
+  // droplet gather records {u, 0} and {v, n_stage} per node (z-quad layout,
+  // recU / recV), one array per stage parity: the density chain of stage s
+  // reads grec[s & 1] while the flow chain of stage s + 1 writes the other
+  V4<T>* grec[2] = {nullptr, nullptr};
   T *v = nullptr, *n_state = nullptr, *n_stage = nullptr, *nacc = nullptr, *fc = nullptr;
   double *partials = nullptr, *partials2 = nullptr, *clip_stage = nullptr, *clip_end = nullptr, *clip_none = nullptr;
   double *maxpart = nullptr, *ek = nullptr, *fac = nullptr, *lastmax = nullptr;
@@ -293,7 +295,7 @@ struct Engine final : EngineBase {
     // per-realization state and work buffers (one slab per realization)
     arena(uk, 3 * CF); arena(us, 3 * CF); arena(acc, 3 * CF); arena(w6, 6 * CF); arena(wn, 4 * CF);
     arena(ubuf[0], 3 * RF); arena(ubuf[1], 3 * RF); arena(v, 3 * RF);
-    arena(gu[0], RF); arena(gu[1], RF); arena(gv, RF);
+    arena(grec[0], 2 * RF); arena(grec[1], 2 * RF);
     arena(n_state, RF); arena(n_stage, RF); arena(nacc, RF); arena(fc, RF);
     const int npart = NBLK_Z1 > NBLK_SUM ? NBLK_Z1 : NBLK_SUM;
     arena(partials, 2 * (size_t)npart); arena(partials2, 2 * (size_t)npart);
@@ -575,7 +577,7 @@ struct Engine final : EngineBase {
   int stage_b(int s, double h) {
     const C<T>* usp = (s == 0) ? uk : us;
     const T* ns = (s == 0) ? n_state : n_stage;
-    V4<T>* gup = gu[s & 1];
+    V4<T>* gup = grec[s & 1];
     const double wh = (s == 0 || s == 3) ? h / 6.0 : h / 3.0;   // forced-dns-sm.py:342-358
     const double anext = a_coef[s + 1] * h;
     LD_TRY(nonlinear_yz(st, nullptr, gup, band ? kc : N / 2));
@@ -589,10 +591,10 @@ struct Engine final : EngineBase {
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XRHS, st));
     // v, nv need the previous density chain done (n_s, clip, wn and v free)
-    return velocity(st, nullptr, gv, ns, clip_stage, evB);
+    return velocity(st, nullptr, gup, ns, clip_stage, evB);
   }
   int stage_d(int s, double h) {
-    V4<T>* gup = gu[s & 1];
+    V4<T>* gup = grec[s & 1];
     const double wh = (s == 0 || s == 3) ? h / 6.0 : h / 3.0;
     const double anext = a_coef[s + 1] * h;
     LD_TRY(join(st2, st, evA));
@@ -603,7 +605,7 @@ struct Engine final : EngineBase {
       PStage<T, PT> ps;
       ps.np = (int)np; ps.cap = (int)cap; ps.stage = s;
       ps.x0 = x0; ps.v0 = v0; ps.sx = sx; ps.sv = sv; ps.kx = kxp; ps.kv = kvp;
-      ps.gu = gup; ps.gv = gv; ps.dep = dep; ps.flist = flist; ps.fcount = fcount;
+      ps.gu = gup; ps.gv = gup; ps.dep = dep; ps.flist = flist; ps.fcount = fcount;
       ps.all_fallback = sorted_ok ? 0 : 1;
       ps.wmax = wmaxp;
       ps.ah = a_coef[s] * h; ps.wh = wh; ps.P = dp; ps.rs = rs;

@@ -394,6 +394,24 @@ struct OpZFwd {
   }
 };
 
+// Droplet gather records.  Per node two 16-byte records, {u_x, u_y, u_z, 0}
+// and {v_x, v_y, v_z, n_stage}, laid out in z-quads: the four u records of
+// nodes z = 4q .. 4q+3 of an (x, y) line (64 bytes), then their four v records,
+// so a 128-byte line holds both records of four consecutive z nodes and a
+// droplet's 4-node z run of both records touches one or two lines instead of
+// two to four.  Node index o = (x N + y) N + z.
+__device__ __forceinline__ uint32_t recU(uint32_t o) { return ((o >> 2) << 3) | (o & 3u); }
+__device__ __forceinline__ uint32_t recV(uint32_t o) { return recU(o) | 4u; }
+// Load of record idx: a 32-bit byte offset from the uniform base where the
+// array (2 N^3 records) fits in 4 GiB, else a 64-bit one (512^3 FP64).
+template <typename T, int N>
+__device__ __forceinline__ V4<T> ldRec(const V4<T>* base, uint32_t idx) {
+  if constexpr (2ull * N * N * N * sizeof(V4<T>) <= (1ull << 32))
+    return ld32(base, idx);
+  else
+    return base[idx];
+}
+
 // Velocity and vorticity to physical space, u stored, then the rotational
 // nonlinear term u x omega transformed along z (forced-dns-sm.py:291-301, and
 // the first pass of the r2c at :304-306).  Spectral input: 6 fields
@@ -401,7 +419,7 @@ struct OpZFwd {
 // 3 fields written over inputs 0..2.
 template <typename T, int N, int P>
 struct OpZCross {
-  C<T>* w; size_t fs; T* u; V4<T>* gu; T scale;   // u planar (may be null) and/or node-interleaved (may be null)
+  C<T>* w; size_t fs; T* u; V4<T>* gu; T scale;   // u planar (may be null) and/or gather records (may be null)
   __device__ OpZCross shifted(size_t rs) const {
     OpZCross o = *this; o.w = rsel1(w, rs); o.u = rsel(u, rs); o.gu = rsel(gu, rs); return o;
   }
@@ -420,7 +438,7 @@ struct OpZCross {
       const T wx = t.re(3, p, j) * scale, wy = t.re(4, p, j) * scale, wz = t.re(5, p, j) * scale;
       const size_t o = (size_t)(line0 + p) * N + j;
       if (u) { u[o] = ux; u[R + o] = uy; u[2 * R + o] = uz; }
-      if (gu) { V4<T> q; q.x = ux; q.y = uy; q.z = uz; q.w = (T)0; gu[o] = q; }
+      if (gu) { V4<T> q; q.x = ux; q.y = uy; q.z = uz; q.w = (T)0; gu[recU((uint32_t)o)] = q; }
       t.re(0, p, j) = wz * uy - wy * uz;
       t.re(1, p, j) = wx * uz - wz * ux;
       t.re(2, p, j) = wy * ux - wx * uy;
@@ -474,7 +492,7 @@ struct OpZVel {
       const T vz = t.re(2, p, j) * scale - tpsg;
       const T nn = nStage<T>(nq[it], clip);
       if (v) { v[o] = vx; v[R + o] = vy; v[2 * R + o] = vz; }
-      if (gv) { V4<T> q; q.x = vx; q.y = vy; q.z = vz; q.w = nn; gv[o] = q; }
+      if (gv) { V4<T> q; q.x = vx; q.y = vy; q.z = vz; q.w = nn; gv[recV((uint32_t)o)] = q; }
       t.re(0, p, j) = nn * vx;
       t.re(1, p, j) = nn * vy;
       t.re(2, p, j) = nn * vz;

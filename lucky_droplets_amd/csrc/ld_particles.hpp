@@ -279,7 +279,7 @@ template <typename T, typename PT> struct PStage {
   double* x0; PT* v0;          // RK base (unwrapped); final stage writes the new state here
   double* sx; PT* sv;          // sump accumulator
   double* kx; PT* kv;          // stage rhs (read: previous stage, write: this stage)
-  const V4<T>* gu; const V4<T>* gv;   // {u, 0} and {v, n_stage} per node
+  const V4<T>* gu; const V4<T>* gv;   // gather records {u, 0} (recU) and {v, n_stage} (recV); one array
   double4* dep;                // out: wrapped stage position and W per slot (brick deposit input)
   int* flist; int* fcount;     // out: slots whose stage bin moved > 1 bin since the sort
   int all_fallback;            // 1: slots are not bin-sorted, every slot takes the fallback deposit
@@ -391,8 +391,8 @@ k_particles_stage(PStage<T, PT> A0) {
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const uint32_t o = (plane + (uint32_t)iy[b]) * N + (uint32_t)iz;
-        U[a * 4 + b] = ld32(A.gu, o);
-        V[a * 4 + b] = ld32(A.gv, o);
+        U[a * 4 + b] = ldRec<T, N>(A.gu, recU(o));
+        V[a * 4 + b] = ldRec<T, N>(A.gv, recV(o));
       }
     }
 #pragma unroll
@@ -421,8 +421,8 @@ k_particles_stage(PStage<T, PT> A0) {
       for (int b = 0; b < 4; ++b) {
         const uint32_t o = (plane + (uint32_t)iy[b]) * N + (uint32_t)iz;   // < N^3 <= 2^27
         const GT w = wy[b] * wx;
-        const V4<T> U = ld32(A.gu, o);
-        const V4<T> V = ld32(A.gv, o);
+        const V4<T> U = ldRec<T, N>(A.gu, recU(o));
+        const V4<T> V = ldRec<T, N>(A.gv, recV(o));
         Ig[0] += (GT)U.x * w;
         Ig[1] += (GT)U.y * w;
         Ig[2] += (GT)U.z * w;
