@@ -192,25 +192,45 @@ def droplet_launch_bytes(N, s, npart, sp):
     return min(7 * R, 448 * s * npart) + R + 5 * npart * sp
 
 
-def cpu_baseline(uk, n, coord, cfg, dt):
-    """The CPU FP64 oracle (numpy + scipy.fft) on the same inputs, bounded
-    sample: one full_RHS evaluation and one step epilogue, scaled to steps/s
-    as 1 / (4 t_rhs + t_epilogue)."""
+def cpu_baseline(uk, n, coord, cfg, dt, c1_steps=10):
+    """The CPU FP64 oracle (numpy + scipy.fft) timed on this host, bounded:
+    one full step (RK4 + epilogue) of the benchmark workload on the same
+    inputs (the reported value, steps/s), and c1_steps full steps of the 64^3
+    workload with one scipy.fft worker (the reference's per-rank FFT, which
+    passes no workers) and with min(16, cpu_count) workers."""
+    import numpy as np
     import oracle
     import oracle.flow as oflow
-    workers = min(16, os.cpu_count() or 1)
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.config import preset
+    ncpu = os.cpu_count() or 1
+    workers = min(16, ncpu)
+
+    def flow_drops(c, crd):
+        fl = oracle.Flow(c.N, c.nu_eff, c.shell_no, c.f0, c.tps, c.g, c.isforcing, 1.0 if c.explicit else 0.0,
+                         c.lp, c.L)
+        dr = oracle.Droplets(c.L, c.N, crd.copy(), c.sts, c.g, c.nu_eff, c.tf, c.rhop, c.M0)
+        return fl, dr
+
+    c1 = preset("c1")
+    rng = np.random.default_rng(c1.seed)
+    uk1, n1, u1 = ic.initial_flow(c1, rng)
+    dt1 = ic.cfl_dt(c1, u1)
+    coord1, _ = ic.initial_droplets(c1, rng)
+    c1_times = {}
+    for w in (1, workers):
+        oflow.WORKERS = w
+        fl, dr = flow_drops(c1, coord1)
+        a, b = uk1.copy(), n1.copy()
+        t = time.time()
+        for i in range(c1_steps):
+            a, b = fl.step(i * dt1, dt1, dr, a, b)
+        c1_times[w] = (time.time() - t) / c1_steps
     oflow.WORKERS = workers
-    fl = oracle.Flow(cfg.N, cfg.nu_eff, cfg.shell_no, cfg.f0, cfg.tps, cfg.g, cfg.isforcing,
-                     1.0 if cfg.explicit else 0.0, cfg.lp, cfg.L)
-    dr = oracle.Droplets(cfg.L, cfg.N, coord.copy(), cfg.sts, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0)
+    fl, dr = flow_drops(cfg, coord)
     t = time.time()
-    fl.full_RHS(0.0, uk, n.copy(), dr.coord.copy(), dr.coord.copy(), dr)
-    t_rhs = time.time() - t
-    t = time.time()
-    fl.clip_zero(n.copy())
-    fl.reality_and_projection(uk)
-    t_epi = time.time() - t
-    step_s = 4 * t_rhs + t_epi
+    fl.step(0.0, dt, dr, uk.copy(), n.copy())
+    step_s = time.time() - t
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -218,9 +238,13 @@ def cpu_baseline(uk, n, coord, cfg, dt):
     except OSError:
         pass
     return {"value": 1.0 / step_s, "unit": "steps/s", "cores": workers, "kind": "port",
-            "sample": f"one full_RHS ({t_rhs:.1f} s) + one epilogue ({t_epi:.1f} s) of the FP64 oracle at "
-                      f"{cfg.N}^3 + {cfg.n_particles} droplets, scipy.fft workers={workers}, "
-                      f"step = 4 RHS + epilogue; host: {model}"}
+            "sample": f"one full step (RK4: 4 full_RHS + droplets, epilogue) of the FP64 oracle at {cfg.N}^3 + "
+                      f"{cfg.n_particles} droplets on the same inputs: {step_s:.1f} s, scipy.fft workers={workers}; "
+                      f"host: {model}, os.cpu_count()={ncpu}",
+            "c1_64cubed_1e4_droplets": {"s_per_step_workers_1": c1_times[1], f"s_per_step_workers_{workers}":
+                                        c1_times[workers], "steps_timed": c1_steps,
+                                        "est_200_steps_s_workers_1": 200 * c1_times[1]},
+            "cpu_count": ncpu}
 
 
 def main():
@@ -353,7 +377,7 @@ def main():
         log(f"[bench] CNN forward {msf:.3f} ms, train step {mst:.3f} ms")
     out["blowup"] = bool(blown)
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle, bounded sample)")
+        log("[bench] CPU baseline (oracle: one full step at the benchmark workload, 10 steps at 64^3)")
         out["cpu_baseline"] = cpu_baseline(uk, n, coord, cfg, dt)
         log(f"[bench] CPU {out['cpu_baseline']['value']:.4f} steps/s")
     print(json.dumps(out), flush=True)

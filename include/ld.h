@@ -149,6 +149,15 @@ int ld_status_all(ld_ctx* ctx, double* last_max, int32_t* blown_up);
  * counterpart in the reference. */
 int ld_tuning(ld_ctx* ctx, int32_t* ylines, double* us);
 
+/* Every launch-geometry choice (host arrays): choice[6] = y-pass lines per
+ * block, z-tile height of the u x omega pass and of the v pass, lines per
+ * block of the u/omega, RHS and density x passes; us[13] = the y-pass
+ * candidates' median times (as ld_tuning), then per tuned kernel the times of
+ * its default and its alternative (half the height or lines; 0 where not
+ * timed).  Every choice gives the same results.  No counterpart in the
+ * reference. */
+int ld_tuning_all(ld_ctx* ctx, int32_t* choice, double* us);
+
 /* ---------------------------------------------------------- measurement */
 /* Kernel classes that ld_profile can time (HIP events recorded on the stream
  * each launch runs on).  No counterpart in the reference, which only prints

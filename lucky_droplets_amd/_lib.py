@@ -106,6 +106,7 @@ _SIGS = {
     "ld_pair_stats": [_VP, _VP, ctypes.c_int64, ctypes.c_double, ctypes.c_int32, _VP, _VP, _VP],
     "ld_status_all": [_VP, _VP, _VP],
     "ld_tuning": [_VP, _VP, _VP],
+    "ld_tuning_all": [_VP, _VP, _VP],
     "ld_cnn_create": [ctypes.POINTER(LdCnnParams), ctypes.POINTER(_VP)],
     "ld_cnn_destroy": [_VP],
     "ld_cnn_set_stream": [_VP, _VP],

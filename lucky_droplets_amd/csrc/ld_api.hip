@@ -252,6 +252,11 @@ int ld_tuning(ld_ctx* ctx, int32_t* ylines, double* us) {
   return ctx->eng->tuning(ylines, us);
 }
 
+int ld_tuning_all(ld_ctx* ctx, int32_t* choice, double* us) {
+  LD_CHECK_CTX(ctx);
+  return ctx->eng->tuning_all(choice, us);
+}
+
 int ld_status_all(ld_ctx* ctx, double* last_max, int32_t* blown_up) {
   LD_CHECK_CTX(ctx);
   return ctx->eng->status_all(last_max, blown_up);

@@ -69,6 +69,16 @@ struct Engine final : EngineBase {
   static constexpr int NBLK_X3 = N * Nf / M3;
   static constexpr int M4 = XM4<T, N, M>::v;         // lines per block of the four-field x pass
   static constexpr int NBLK_X4 = N * Nf / M4;
+  // the autotuner's alternatives: half the z-tile height (>= 2) and half the
+  // x-pass lines per block (>= 2); a block's lines do not change any line's
+  // arithmetic, so every choice gives the same bits
+  static constexpr int P6H = P6 / 2 >= 2 ? P6 / 2 : P6, P3H = P3 / 2 >= 2 ? P3 / 2 : P3;
+  static constexpr int M3H = M3 / 2 >= 2 ? M3 / 2 : M3, M4H = M4 / 2 >= 2 ? M4 / 2 : M4;
+  // launch-geometry choices: 0 = default, 1 = the alternative above, for the
+  // u x omega z pass, the v z pass, the u/omega x pass, the RHS x pass and the
+  // density x pass (set by the autotuner or LD_TUNE_FORCE=a,b,c,d,e)
+  int vsel[5] = {0, 0, 0, 0, 0};
+  double vus[5][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
   static constexpr int NBLK_SUM = 1024;             // blocks of k_sum2
 
   ld_params p;
@@ -189,9 +199,58 @@ struct Engine final : EngineBase {
   int batch() const override { return B; }
 
   // Autotuner: time the 6-field y-inverse pass with M, M/2 and 2M lines per
-  // block (after one warm-up launch each, the median of five) and keep the
-  // fastest.
+  // block, then each of the z passes of u x omega and v with its default and
+  // half tile height and each of the three x passes with its default and half
+  // lines per block (on the context's zeroed work buffers, as the truncated
+  // step launches them; after one warm-up launch each, the median of five)
+  // and keep the fastest of each.
   double tune_us[3] = {0.0, 0.0, 0.0};
+  template <class F> int time_us(hipEvent_t e0, hipEvent_t e1, F&& launch, double& us) {
+    std::vector<float> t;
+    for (int r = 0; r < 6; ++r) {
+      LD_HIP(hipEventRecord(e0, st));
+      LD_TRY(launch());
+      LD_HIP(hipEventRecord(e1, st));
+      LD_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      LD_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (r > 0) t.push_back(ms);
+    }
+    std::sort(t.begin(), t.end());
+    us = 1000.0 * t[t.size() / 2];
+    return LD_OK;
+  }
+  int autotune_geometry(hipEvent_t e0, hipEvent_t e1) {
+    LD_TRY(fill<C<T>>(st, wn, 4 * CF, C<T>{}));
+    LD_TRY(fill<C<T>>(st, us, 3 * CF, C<T>{}));
+    LD_TRY(fill<C<T>>(st, uk, 3 * CF, C<T>{}));
+    LD_TRY(fill<C<T>>(st, acc, 3 * CF, C<T>{}));
+    LD_TRY(fill<T>(st, n_state, RF, (T)0));
+    const bool alt[5] = {P6H != P6, P3H != P3, M3H != M3, M3H != M3, M4H != M4};
+    RhsArgs<T> ra;
+    ra.nl = w6; ra.us = us; ra.ubase = uk; ra.acc = acc; ra.usnext = us; ra.ku = nullptr;
+    ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
+    ra.wacc = (T)0; ra.anext = (T)0; ra.scale = inv(); ra.acc_init = 0; ra.kcut = kc; ra.band = kc < N / 2 ? 1 : 0;
+    for (int k = 0; k < 5; ++k) {
+      for (int v = 0; v < 2; ++v) {
+        vus[k][v] = 0.0;
+        if (v == 1 && !alt[k]) continue;
+        vsel[k] = v;
+        auto launch = [&]() -> int {
+          switch (k) {
+            case 0: return zcross(st, nullptr, grec[0], 0, N);
+            case 1: return zvel(st, nullptr, grec[0], n_state, clip_none, 0, N);
+            case 2: return nonlinear_x(st, uk, kc);
+            case 3: return xrhs(st, ra);
+            default: return xkn(st);
+          }
+        };
+        LD_TRY(time_us(e0, e1, launch, vus[k][v]));
+      }
+      vsel[k] = (alt[k] && vus[k][1] > 0.0 && vus[k][1] < vus[k][0]) ? 1 : 0;
+    }
+    return LD_OK;
+  }
   int autotune() {
     LD_TRY(fill<C<T>>(st, w6, 6 * CF, C<T>{}));
     hipEvent_t e0, e1;
@@ -215,12 +274,25 @@ struct Engine final : EngineBase {
       std::sort(t.begin(), t.end());
       tune_us[c] = 1000.0 * t[t.size() / 2];
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     int best = 0;
     for (int c = 1; c < 3; ++c)
       if (tune_us[c] > 0.0 && tune_us[c] < tune_us[best]) best = c;
     my = cand[best];
+    const int rc = autotune_geometry(e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+  }
+  int tuning_all(int32_t* choice, double* us_out) override {
+    const int32_t sizes[5][2] = {{P6, P6H}, {P3, P3H}, {M3, M3H}, {M3, M3H}, {M4, M4H}};
+    if (choice) {
+      choice[0] = my;
+      for (int k = 0; k < 5; ++k) choice[1 + k] = sizes[k][vsel[k]];
+    }
+    if (us_out) {
+      for (int c = 0; c < 3; ++c) us_out[c] = tune_us[c];
+      for (int k = 0; k < 5; ++k) { us_out[3 + 2 * k] = vus[k][0]; us_out[4 + 2 * k] = vus[k][1]; }
+    }
     return LD_OK;
   }
   int tuning(int32_t* ylines, double* us) override {
@@ -338,6 +410,14 @@ struct Engine final : EngineBase {
     dp.inv_dx = (double)N / p.L;
     dp.inv_factor = 1.0 / p.factor;
     if (p.autotune) LD_TRY(autotune());
+    // LD_TUNE_FORCE=a,b,c,d,e (0/1 each): force the geometry choices (measurement, tests)
+    if (const char* e = getenv("LD_TUNE_FORCE")) {
+      for (int k = 0; k < 5 && *e; ++k) {
+        vsel[k] = (*e == '1') ? 1 : 0;
+        while (*e && *e != ',') ++e;
+        if (*e == ',') ++e;
+      }
+    }
     return LD_OK;
   }
 
@@ -498,19 +578,30 @@ struct Engine final : EngineBase {
   // kcu < N/2: the stage u_hat has no nonzero mode outside the mask
   int nonlinear_x(hipStream_t s, const C<T>* usp, int kcu = N / 2) {
     LD_TRY(prof_mark(LD_K_XUW, s));
-    hipLaunchKernelGGL((k_x_uw3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, s, usp, w6, CF, (const C<T>*)tw,
-                       inv(), rs, kcu);
+    if (vsel[2])
+      hipLaunchKernelGGL((k_x_uw3<T, N, M3H>), dim3(N * Nf / M3H, 1, B), dim3(3 * M3H * N1), 0, s, usp, w6, CF,
+                         (const C<T>*)tw, inv(), rs, kcu);
+    else
+      hipLaunchKernelGGL((k_x_uw3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, s, usp, w6, CF,
+                         (const C<T>*)tw, inv(), rs, kcu);
     LD_LAUNCHED();
     LD_TRY(prof_mark(LD_K_XUW, s));
     return LD_OK;
   }
   // kcu < N/2: the stage u_hat has no nonzero mode outside the mask
-  int nonlinear_yz(hipStream_t s, T* uphys, V4<T>* g, int kcu = N / 2) {
+  int zcross(hipStream_t s, T* uphys, V4<T>* g, int x0, int nx) {
+    if (vsel[0]) {
+      OpZCross<T, N, P6H> oc{w6, CF, uphys, g, inv()};
+      return zrun<P6H, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc);
+    }
     OpZCross<T, N, P6> oc{w6, CF, uphys, g, inv()};
+    return zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc);
+  }
+  int nonlinear_yz(hipStream_t s, T* uphys, V4<T>* g, int kcu = N / 2) {
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx, kcu, kcu < N / 2 ? sphere() : -1)));
-      LD_TRY((zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc)));
+      LD_TRY(zcross(s, uphys, g, x0, nx));
       LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx, kc, sphere())));
     }
     return LD_OK;
@@ -519,14 +610,40 @@ struct Engine final : EngineBase {
   // z-forward into wn[0..3).
   // v_hat y-inverse and z pass (v out, n v forward) slab by slab; `wait`
   // (may be null) is waited for before the first z pass writes v / nv.
-  int velocity(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip, hipEvent_t wait) {
+  int zvel(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip, int x0, int nx) {
+    if (vsel[1]) {
+      OpZVel<T, N, P3H> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
+      return zrun<P3H, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc);
+    }
     OpZVel<T, N, P3> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
+    return zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc);
+  }
+  int velocity(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip, hipEvent_t wait) {
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, w6, w6, 3, inv(), x0, nx, kc, sphere())));
       if (x0 == 0 && wait) LD_HIP(hipStreamWaitEvent(s, wait, 0));
-      LD_TRY((zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc)));
+      LD_TRY(zvel(s, vphys, g, ns, clip, x0, nx));
     }
+    return LD_OK;
+  }
+  int xkn(hipStream_t s) {
+    if (vsel[4])
+      hipLaunchKernelGGL((k_x_kn4<T, N, M4H>), dim3(N * Nf / M4H, 1, B), dim3(4 * M4H * N1), 0, s, wn, CF,
+                         (const C<T>*)tw, inv(), rs, kc);
+    else
+      hipLaunchKernelGGL((k_x_kn4<T, N, M4>), dim3(NBLK_X4, 1, B), dim3(4 * M4 * N1), 0, s, wn, CF, (const C<T>*)tw,
+                         inv(), rs, kc);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  int xrhs(hipStream_t s, const RhsArgs<T>& ra) {
+    if (vsel[3])
+      hipLaunchKernelGGL((k_x_rhs3<T, N, M3H>), dim3(N * Nf / M3H, 1, B), dim3(3 * M3H * N1), 0, s, ra,
+                         (const C<T>*)tw, rs);
+    else
+      hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, s, ra, (const C<T>*)tw, rs);
+    LD_LAUNCHED();
     return LD_OK;
   }
   // Density chain on stream s: fc -> z-forward into wn[3]; y-forward of
@@ -541,9 +658,7 @@ struct Engine final : EngineBase {
       LD_TRY((pass<1, -1>(s, wn, wn, 4, (T)1, x0, nx, kc, sphere())));
     }
     LD_TRY(prof_mark(LD_K_XKN, s));
-    hipLaunchKernelGGL((k_x_kn4<T, N, M4>), dim3(NBLK_X4, 1, B), dim3(4 * M4 * N1), 0, s, wn, CF, (const C<T>*)tw,
-                       inv(), rs, kc);
-    LD_LAUNCHED();
+    LD_TRY(xkn(s));
     LD_TRY(prof_mark(LD_K_XKN, s));
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, wn, wn, 1, inv(), x0, nx, kc, sphere())));
@@ -587,8 +702,7 @@ struct Engine final : EngineBase {
     ra.wacc = (T)wh; ra.anext = (T)anext; ra.scale = inv(); ra.acc_init = (s == 0);
     ra.kcut = kc; ra.band = band ? 1 : 0;
     LD_TRY(prof_mark(LD_K_XRHS, st));
-    hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw, rs);
-    LD_LAUNCHED();
+    LD_TRY(xrhs(st, ra));
     LD_TRY(prof_mark(LD_K_XRHS, st));
     // v, nv need the previous density chain done (n_s, clip, wn and v free)
     return velocity(st, nullptr, gup, ns, clip_stage, evB);
@@ -932,8 +1046,7 @@ struct Engine final : EngineBase {
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit * visc); ra.tps = (T)p.tps;
     ra.wacc = (T)0; ra.anext = (T)0; ra.scale = inv(); ra.acc_init = 0;
     ra.kcut = kc; ra.band = 0;
-    hipLaunchKernelGGL((k_x_rhs3<T, N, M3>), dim3(NBLK_X3, 1, B), dim3(3 * M3 * N1), 0, st, ra, (const C<T>*)tw, rs);
-    LD_LAUNCHED();
+    LD_TRY(xrhs(st, ra));
     // the physical-space v the droplets see uses visc as well (forced-dns-sm.py:319-321)
     LD_TRY(velocity(st, vphys, nullptr, (const T*)nin, clip_none, nullptr));
     LD_TRY(fill<T>(st, fcp, RF, (T)0));

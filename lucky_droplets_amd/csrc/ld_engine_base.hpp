@@ -28,6 +28,7 @@ struct EngineBase {
   virtual int status_all(double* last_max, int32_t* blown) = 0;
   virtual int batch() const = 0;
   virtual int tuning(int32_t* ylines, double* us) = 0;
+  virtual int tuning_all(int32_t* choice, double* us) = 0;
   virtual int interp(const double* coord, int64_t np, const void* f, int nc, double* out) = 0;
   virtual int exterp(const double* coord, int64_t np, const double* w, void* c) = 0;
   virtual int collide(const double* rows, const double* rad, int64_t np, double rmax, int64_t* pairs,

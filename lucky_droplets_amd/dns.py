@@ -260,6 +260,17 @@ class ForcedDNS:
         _lib.check(self.lib.ld_tuning(self.ctx, ctypes.byref(yl), us))
         return yl.value, [us[0], us[1], us[2]]
 
+    def tuning_all(self):
+        """Every launch-geometry choice: (dict of choices, dict of timings in us)."""
+        ch = (ctypes.c_int32 * 6)()
+        us = (ctypes.c_double * 13)()
+        _lib.check(self.lib.ld_tuning_all(self.ctx, ch, us))
+        names = ["y_lines", "z_cross_tile", "z_vel_tile", "x_uw_lines", "x_rhs_lines", "x_kn_lines"]
+        times = {"y_lines": list(us[0:3])}
+        for k, nm in enumerate(names[1:]):
+            times[nm] = [us[3 + 2 * k], us[4 + 2 * k]]
+        return dict(zip(names, ch[:])), times
+
     def status_all(self):
         """(last max Re(u_hat), blown-up flag) per realization, as numpy arrays."""
         import numpy as np

@@ -656,6 +656,41 @@ def test_autotune_keeps_results(gpu):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+@pytest.mark.parametrize("N,prec", [(64, "f32"), (32, "f64"), (256, "f32")])
+def test_every_launch_geometry_gives_the_same_bits(gpu, N, prec, monkeypatch):
+    """The z-tile heights and x-pass line counts the autotuner chooses between
+    (defaults, and all halved at once) give bit-identical steps; the autotuner
+    reports a choice and a time for every candidate it ran."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(N, prec, 3000 if N < 256 else 20000, nu=0.01)
+    rng = np.random.default_rng(31)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, _ = ic.initial_droplets(cfg, rng)
+    outs = []
+    for force in ("0,0,0,0,0", "1,1,1,1,1"):
+        monkeypatch.setenv("LD_TUNE_FORCE", force)
+        d = ForcedDNS(cfg)
+        t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+        d.set_state(*t)
+        d.advance(dt, 2)
+        d.get_state(*t)
+        torch.cuda.synchronize()
+        outs.append([x.cpu().numpy() for x in t])
+        d.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    monkeypatch.delenv("LD_TUNE_FORCE")
+    d = ForcedDNS(cfg, autotune=True)
+    choice, times = d.tuning_all()
+    assert choice["y_lines"] in (4, 8, 16, 32)
+    for k in ("z_cross_tile", "z_vel_tile", "x_uw_lines", "x_rhs_lines", "x_kn_lines"):
+        assert choice[k] >= 2 and times[k][0] > 0
+    d.close()
+
+
 def test_c1_200_steps_against_oracle_golden(gpu):
     """The 64^3 FP64 workload (10^4 droplets, 200 steps) against the committed
     CPU-oracle run (tests/golden/make_c1_golden.py): 1e-10 relative on the
