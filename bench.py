@@ -185,6 +185,36 @@ def time_plain(cfg, uk, n, coord, dt, steps):
     return e0.elapsed_time(e1) / steps
 
 
+def time_collisions(cfg, coord, reps=5):
+    """ms per cell-list collision-pair search and per pair-statistics pass over
+    the workload's droplets (radius r = sqrt(9 nu tau_eta St / (2 rho_p)))."""
+    import torch
+    from lucky_droplets_amd.dns import ForcedDNS
+    from lucky_droplets_amd.particles import MPI_particles
+    dev = torch.device("cuda:0")
+    d = ForcedDNS(cfg)
+    stb = MPI_particles(None, cfg.L, cfg.N, cfg.n_particles, cfg.sts, 0.3, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0,
+                        dns=d, coord=torch.from_numpy(coord).to(dev))
+    rad = stb.radius(cfg.nu_eff, cfg.rhop)
+    npairs = stb.collision_pairs(rad).shape[0]
+    stb.pair_statistics(0.1, 64)
+    torch.cuda.synchronize()
+    t = time.time()
+    for _ in range(reps):
+        stb.collision_pairs(rad)
+    torch.cuda.synchronize()
+    ms_pairs = (time.time() - t) / reps * 1e3
+    t = time.time()
+    for _ in range(reps):
+        stb.pair_statistics(0.1, 64)
+    torch.cuda.synchronize()
+    ms_stats = (time.time() - t) / reps * 1e3
+    d.close()
+    return {"droplets": cfg.n_particles, "pairs": int(npairs), "ms_collision_pairs": ms_pairs,
+            "ms_pair_statistics_rmax_0.1": ms_stats,
+            "note": "host wall clock per call (each call synchronises to read its count)"}
+
+
 def droplet_launch_bytes(N, s, npart, sp):
     """Algorithmic bytes of one droplet-stage launch (one quarter of the
     per-step droplet terms): min(7R, 448 s Np) + R + 5 Np S_p."""
@@ -351,6 +381,11 @@ def main():
         out["fp32_accuracy"] = rep
         log(f"[bench] spectrum error max {rep['spectrum_err_max']:.2e} (shell {rep['spectrum_err_max_shell']}), "
             f"dissipation error {rep['dissipation_err']:.2e}")
+    if cfg.n_particles:
+        log("[bench] collision pairs and pair statistics")
+        out["collisions"] = time_collisions(cfg, coord)
+        log(f"[bench] collisions {out['collisions']['ms_collision_pairs']:.2f} ms, "
+            f"{out['collisions']['pairs']} pairs")
     if not args.no_ensemble:
         from lucky_droplets_amd.config import preset as _preset
         c3 = _preset("c3")

@@ -103,3 +103,26 @@ def coalesce(rows: np.ndarray, pairs: np.ndarray, L: float):
         used[i] = used[j] = True
         events.append((i, j))
     return rows, np.array(events, dtype=np.int64).reshape(-1, 2)
+
+
+def pairs_kdtree(pos: np.ndarray, radius: np.ndarray, L: float) -> np.ndarray:
+    """The pairs of pairs_bruteforce found with an independent periodic
+    neighbour search (scipy.spatial.cKDTree with boxsize L) for the candidates
+    within 2 max(r) (slightly widened), then the same FP64 test on each
+    candidate; for populations too large for the O(Np^2) loop."""
+    from scipy.spatial import cKDTree
+    x = wrap(np.asarray(pos, dtype=np.float64)[:, :3], L)
+    r = np.asarray(radius, dtype=np.float64)
+    xt = np.where(x >= L, x - L, x)                # the tree needs [0, L)
+    cand = cKDTree(xt, boxsize=L).query_pairs(2.0 * r.max() * (1 + 1e-9), output_type="ndarray")
+    if cand.size == 0:
+        return np.zeros((0, 2), dtype=np.int64)
+    i = np.minimum(cand[:, 0], cand[:, 1]).astype(np.int64)
+    j = np.maximum(cand[:, 0], cand[:, 1]).astype(np.int64)
+    d = x[j] - x[i]
+    d = d - L * np.rint(d / L)
+    d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    s = r[i] + r[j]
+    keep = d2 < s * s
+    out = np.stack([i[keep], j[keep]], axis=1)
+    return out[np.lexsort((out[:, 1], out[:, 0]))]

@@ -92,3 +92,18 @@ def test_pair_stats_matches_pair_test():
     rmax = 0.7
     c, _, _ = collide.pair_stats_bruteforce(rows, L, rmax, 10)
     assert c.sum() == collide.pairs_bruteforce(rows[:, :3], np.full(500, rmax / 2), L).shape[0]
+
+
+def test_kdtree_pairs_equal_bruteforce():
+    """The periodic k-d tree search (used at 10^5 droplets) finds exactly the
+    brute-force pairs, seam crossings included."""
+    rng = np.random.default_rng(12)
+    L = 2 * np.pi
+    n = 2500
+    pos = rng.uniform(0, L, (n, 3)) + L * rng.integers(-1, 2, (n, 3))
+    pos[: n // 5, 1] = rng.uniform(-2e-3, 2e-3, n // 5)
+    r = rng.uniform(0.01, 0.08, n)
+    a = collide.pairs_bruteforce(pos, r, L)
+    b = collide.pairs_kdtree(pos, r, L)
+    assert a.shape[0] > 50
+    assert np.array_equal(a, b)
