@@ -49,9 +49,24 @@ def run_dirs(root, t: float, wg: str, sts_ratio: float, stb0: float, with_drople
     return base, ndir, sdir
 
 
+def fluid_velocity_at_droplets(dns, stb, uk):
+    """umat of the state files: the fluid velocity u = irfft(uk) interpolated at
+    the droplets with the cosine kernel (forced-dns-sm.py:455-460), on the GPU;
+    (Np, 3) float64 tensor in the row order of stb.coord."""
+    import torch
+    u = dns.irfft_mpi(uk, dns.zeros_real(3))
+    return stb.interp_cosine(stb.coord, u)[:, :3].clone()
+
+
 def save(root, t, uk, n, *, ranks: int = 1, coord=None, prtclid=None, umat=None, ek=None, pik=None,
-         wg: str = "wo_g", sts_ratio: float = 0.001, stb0: float = 0.05, L: float = 2 * np.pi):
-    """Write one snapshot as `ranks` slabs.  Returns the fields directory."""
+         wg: str = "wo_g", sts_ratio: float = 0.001, stb0: float = 0.05, L: float = 2 * np.pi, dns=None, stb=None):
+    """Write one snapshot as `ranks` slabs.  Returns the fields directory.
+
+    With ``dns`` and ``stb`` (and no ``umat``), the state files' umat is the
+    fluid velocity at the droplets, as the reference writes it; otherwise umat
+    is what the caller passes, or zeros."""
+    if umat is None and dns is not None and stb is not None and coord is not None:
+        umat = fluid_velocity_at_droplets(dns, stb, uk)
     uk = _np(uk)
     n = _np(n)
     N = n.shape[0]
@@ -84,9 +99,24 @@ def save(root, t, uk, n, *, ranks: int = 1, coord=None, prtclid=None, umat=None,
     return base
 
 
-def load_fields(fields_dir, N: int, n_dir: Optional[os.PathLike] = None):
+def dealias_mask(N: int) -> np.ndarray:
+    """The 2/3-rule mask kint <= N/3 on the half spectrum (forced-dns-sm.py:127-129)."""
+    k = np.fft.fftfreq(N, 1.0 / N)
+    kz = np.arange(N // 2 + 1)
+    kk = np.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    kint = np.clip(np.round(kk).astype(int), None, N // 2)
+    return kint <= N / 3
+
+
+def load_fields(fields_dir, N: int, n_dir: Optional[os.PathLike] = None, mask: bool = False):
     """(uk (3, N, N, N/2+1) complex128, n (N, N, N) or ones) re-slabbed from any
-    number of rank files (forced-dns-sm.py:374-407)."""
+    number of rank files (forced-dns-sm.py:374-407).
+
+    mask=True multiplies uk by the dealiasing mask.  The reference's reality
+    round trip leaves round-off (about 1e-16 of the largest mode) in the modes
+    outside the mask, so a state it wrote does not qualify for the stepping's
+    truncated fast path; the mask removes that round-off (the state then
+    differs from the file at that level) and the steps take the fast path."""
     fields_dir = Path(fields_dir)
     nslabs = len([x for x in fields_dir.iterdir() if "Fields" in x.name and x.name.endswith(".npz")])   # :375
     if nslabs == 0:
@@ -102,6 +132,8 @@ def load_fields(fields_dir, N: int, n_dir: Optional[os.PathLike] = None):
         uk[2][:, sl] = F["wk"]
         if n_dir is not None:
             n[sl] = np.load(Path(n_dir) / f"n_{s}.npz")["n"]                                               # :398
+    if mask:
+        uk *= dealias_mask(N)
     return uk, n
 
 

@@ -43,3 +43,20 @@ def test_reads_reference_written_slabs(tmp_path):
         np.savez_compressed(d / "wo_g_sts_0.001_stb_0.050" / f"n_{r}", n=n[s])
     uk2, n2 = checkpoint.load_fields(d, N, d / "wo_g_sts_0.001_stb_0.050")
     assert np.array_equal(uk2, uk) and np.array_equal(n2, n)
+
+
+def test_load_fields_mask_removes_modes_outside_the_dealiasing_mask(tmp_path):
+    """mask=True zeroes exactly the modes with round|k| > N/3 (the mask of
+    forced-dns-sm.py:127-129, as the oracle builds it) and keeps the others."""
+    import oracle
+    N = 16
+    rng = np.random.default_rng(2)
+    uk = rng.standard_normal((3, N, N, 9)) + 1j * rng.standard_normal((3, N, N, 9))
+    d = tmp_path / "f"
+    d.mkdir()
+    np.savez_compressed(d / "Fields_k_0", uk=uk[0], vk=uk[1], wk=uk[2])
+    um, _ = checkpoint.load_fields(d, N, mask=True)
+    keep = oracle.Grid(N).dealias
+    assert np.array_equal(um, uk * keep)
+    assert np.array_equal(checkpoint.dealias_mask(N), keep)
+    assert 0 < keep.sum() < keep.size

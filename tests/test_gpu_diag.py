@@ -72,3 +72,31 @@ def test_droplet_mass_of_resident_state(gpu, prec):
     assert s1 > s
     assert abs(t1 - t0) < (1e-9 if prec == "f64" else 1e-3) * (s1 - s)
     d.close()
+
+
+def test_state_file_umat_is_fluid_velocity_at_droplets(gpu, tmp_path):
+    """save(..., dns=, stb=) writes umat as the reference does
+    (forced-dns-sm.py:455-460): interp_cosine of u at the droplets."""
+    import torch
+    from lucky_droplets_amd import checkpoint
+    from lucky_droplets_amd.dns import ForcedDNS
+    from lucky_droplets_amd.particles import MPI_particles
+    from helpers import make_drops_oracle
+    N, npart = 32, 400
+    cfg = _cfg(N, "f64", npart)
+    uk, n, coord, pid = inputs(cfg, seed=6)
+    d = ForcedDNS(cfg)
+    stb = MPI_particles(None, cfg.L, N, npart, cfg.sts, 0.1, cfg.g, cfg.nu_eff, cfg.tf, cfg.rhop, cfg.M0, dns=d,
+                        coord=coord, prtclid=pid)
+    ukg = _t(uk, gpu, d.cdtype)
+    base = checkpoint.save(tmp_path, 0.5, ukg, _t(n, gpu, d.rdtype), coord=stb.coord, prtclid=stb.prtclid, dns=d,
+                           stb=stb)
+    _, _, sdir = checkpoint.run_dirs(tmp_path, 0.5, "wo_g", 0.001, 0.05, True)
+    st = np.load(sdir / "state_0.npz")
+    G = make_flow_oracle(cfg).grid
+    u = np.stack([G.irfft_mpi(uk[c]) for c in range(3)])
+    dr = make_drops_oracle(cfg, coord, pid)
+    ref = dr.interp_cosine(np.mod(coord, cfg.L), u)
+    order = np.argsort(st["prtclid"].reshape(-1))
+    assert relerr(st["umat"][order], ref[np.argsort(pid)]) < 1e-12
+    d.close()
