@@ -282,6 +282,31 @@ int ld_cnn_grad(ld_cnn* c, const float* x, const float* target, float* grads, do
 int ld_cnn_train_step(ld_cnn* c, const float* x, const float* target, float lr, float beta1, float beta2, float eps,
                       double* loss);
 
+/* Learned-form plumbing (no counterpart in the reference).  All fields are on
+ * the context's grid and precision unless stated.
+ * ld_spectral_resample: nfields half spectra of an n_src^3 grid -> this grid:
+ *   the modes with |kx|, |ky|, kz < min(n_src, N)/2 copied and scaled by
+ *   (N / n_src)^3, all others zero (truncation or zero padding).
+ * ld_velocity_gradient_spec: gk[3 i + j] = scale * i k_j uk_i, the nine
+ *   spectral velocity-gradient components (Nyquist derivatives zero).
+ * ld_fields_to_rows: nfields physical fields f[c][node] -> FP32 rows
+ *   rows[node][c] * scale (the CNN's input layout).
+ * ld_subgrid_compose / ld_subgrid_residual: with ratio r (this grid is the
+ *   fine grid, N / r the coarse one) a coarse node's FP32 row of 3 r^3
+ *   channels holds the 3 velocity components at its r^3 fine sub-nodes
+ *   (channel c r^3 + ((X%r) r + Y%r) r + Z%r):
+ *   compose: out = base + scale * net;  residual: net = scale * (fine - base).
+ * ld_droplets_frozen_rk4: nsteps RK4 steps of size h of droplet rows (np, 7)
+ *   in place in the frozen physical velocity u (3 fields): x' = v,
+ *   v' = (I_u(x) - v) / (St tau_eta) - g e_z, I_u the cosine-kernel gather
+ *   (particles.py:435-511), St = (m / factor)^(2/3) (particles.py:823), m fixed. */
+int ld_spectral_resample(ld_ctx* ctx, const void* src, int32_t n_src, void* dst, int32_t nfields);
+int ld_velocity_gradient_spec(ld_ctx* ctx, const void* uk, void* gk, double scale);
+int ld_fields_to_rows(ld_ctx* ctx, const void* f, int32_t nfields, float* rows, float scale);
+int ld_subgrid_compose(ld_ctx* ctx, const float* net, int32_t ratio, const void* base, void* out, float scale);
+int ld_subgrid_residual(ld_ctx* ctx, const void* fine, const void* base, int32_t ratio, float* net, float scale);
+int ld_droplets_frozen_rk4(ld_ctx* ctx, double* rows, int64_t np, const void* u, double h, int32_t nsteps);
+
 #ifdef __cplusplus
 }
 #endif

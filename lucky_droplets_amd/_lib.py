@@ -107,6 +107,12 @@ _SIGS = {
     "ld_status_all": [_VP, _VP, _VP],
     "ld_tuning": [_VP, _VP, _VP],
     "ld_tuning_all": [_VP, _VP, _VP],
+    "ld_spectral_resample": [_VP, _VP, ctypes.c_int32, _VP, ctypes.c_int32],
+    "ld_fields_to_rows": [_VP, _VP, ctypes.c_int32, _VP, ctypes.c_float],
+    "ld_velocity_gradient_spec": [_VP, _VP, _VP, ctypes.c_double],
+    "ld_subgrid_compose": [_VP, _VP, ctypes.c_int32, _VP, _VP, ctypes.c_float],
+    "ld_subgrid_residual": [_VP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_float],
+    "ld_droplets_frozen_rk4": [_VP, _VP, ctypes.c_int64, _VP, ctypes.c_double, ctypes.c_int32],
     "ld_cnn_create": [ctypes.POINTER(LdCnnParams), ctypes.POINTER(_VP)],
     "ld_cnn_destroy": [_VP],
     "ld_cnn_set_stream": [_VP, _VP],

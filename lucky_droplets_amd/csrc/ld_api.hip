@@ -362,4 +362,51 @@ int ld_prhs(ld_ctx* ctx, const double* coord, int64_t np, const void* u, const v
   return ctx->eng->prhs(coord, np, u, us, n, fc, rhs, interpmat, exterpmat);
 }
 
+static bool pow2(int n) { return n >= 4 && n <= 4096 && (n & (n - 1)) == 0; }
+
+int ld_spectral_resample(ld_ctx* ctx, const void* src, int32_t n_src, void* dst, int32_t nfields) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(src); LD_FIELD(dst);
+  if (!pow2(n_src) || nfields < 1) { set_error("n_src must be a power of two >= 4 and nfields >= 1"); return LD_ERR_ARG; }
+  return ctx->eng->resample(src, n_src, dst, nfields);
+}
+
+int ld_velocity_gradient_spec(ld_ctx* ctx, const void* uk, void* gk, double scale) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(uk); LD_FIELD(gk);
+  return ctx->eng->grad_spec(uk, gk, scale);
+}
+
+int ld_fields_to_rows(ld_ctx* ctx, const void* f, int32_t nfields, float* rows, float scale) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(f); LD_NONNULL(rows);
+  if (nfields < 1 || nfields > 32) { set_error("nfields must be in [1, 32]"); return LD_ERR_ARG; }
+  return ctx->eng->fields_to_rows(f, nfields, rows, scale);
+}
+
+static int subgrid_ratio_ok(ld_ctx* ctx, int32_t r) {
+  if (r < 1 || r > 4 || ctx->p.N % r != 0 || 3 * r * r * r > 192) {
+    set_error("ratio must be in [1, 4] and divide N");
+    return 0;
+  }
+  return 1;
+}
+
+int ld_subgrid_compose(ld_ctx* ctx, const float* net, int32_t ratio, const void* base, void* out, float scale) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_NONNULL(net); LD_FIELD(base); LD_FIELD(out);
+  if (!subgrid_ratio_ok(ctx, ratio)) return LD_ERR_ARG;
+  return ctx->eng->subgrid(net, nullptr, ratio, base, nullptr, out, scale);
+}
+
+int ld_subgrid_residual(ld_ctx* ctx, const void* fine, const void* base, int32_t ratio, float* net, float scale) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(fine); LD_FIELD(base); LD_NONNULL(net);
+  if (!subgrid_ratio_ok(ctx, ratio)) return LD_ERR_ARG;
+  return ctx->eng->subgrid(nullptr, net, ratio, base, fine, nullptr, scale);
+}
+
+int ld_droplets_frozen_rk4(ld_ctx* ctx, double* rows, int64_t np, const void* u, double h, int32_t nsteps) {
+  LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(u);
+  if (np < 0 || nsteps < 0) { set_error("np and nsteps must be >= 0"); return LD_ERR_ARG; }
+  if (np > 0) LD_NONNULL(rows);
+  if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
+  return ctx->eng->frozen_rk4(rows, np, u, h, nsteps);
+}
+
 }  // extern "C"

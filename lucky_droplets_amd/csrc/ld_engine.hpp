@@ -6,6 +6,7 @@
 #include "ld_plain.hpp"
 #include "ld_collide.hpp"
 #include "ld_diag.hpp"
+#include "ld_learned.hpp"
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -1439,6 +1440,42 @@ struct Engine final : EngineBase {
     if (n_ <= 0) return LD_OK;
     hipLaunchKernelGGL((k_exterp<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, coord, (int)n_, w,
                        (T*)c, dp);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  // ------------------------------------------------------------ learned subgrid form
+  int resample(const void* src, int ns, void* dst, int nf) override {
+    LD_TRY(join(st, st2, evB));
+    const double sc = std::pow((double)N / (double)ns, 3.0);
+    hipLaunchKernelGGL((k_resample<T, N>), dim3(2048), dim3(256), 0, st, (const C<T>*)src, ns, (C<T>*)dst, nf, (T)sc);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  int grad_spec(const void* ukin, void* gk, double s) override {
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_grad_spec<T, N>), dim3(2048), dim3(256), 0, st, (const C<T>*)ukin, (C<T>*)gk, (T)s);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  int fields_to_rows(const void* f, int nc, float* rows, float s) override {
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_fields_to_rows<T, N>), dim3(2048), dim3(256), 0, st, (const T*)f, nc, rows, s);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  int subgrid(const float* net_in, float* net_out, int r, const void* base, const void* fine, void* out,
+              float s) override {
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_subgrid<T, N>), dim3(2048), dim3(256), 0, st, net_in, net_out, r, (const T*)base,
+                       (const T*)fine, (T*)out, s);
+    LD_LAUNCHED();
+    return LD_OK;
+  }
+  int frozen_rk4(double* rows, int64_t n_, const void* u, double h, int nsteps) override {
+    if (n_ <= 0 || nsteps <= 0) return LD_OK;
+    LD_TRY(join(st, st2, evB));
+    hipLaunchKernelGGL((k_frozen_rk4<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, rows, (int)n_,
+                       (const T*)u, h, nsteps, dp);
     LD_LAUNCHED();
     return LD_OK;
   }

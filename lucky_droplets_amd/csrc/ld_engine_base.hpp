@@ -49,6 +49,12 @@ struct EngineBase {
   virtual int profile_read(double* total_ms, int64_t* launches) = 0;
   virtual int prhs(const double* coord, int64_t np, const void* u, const void* us, const void* n, void* fc,
                    double* rhs, double* interp, double* exterp) = 0;
+  virtual int resample(const void* src, int ns, void* dst, int nf) = 0;
+  virtual int fields_to_rows(const void* f, int nc, float* rows, float s) = 0;
+  virtual int grad_spec(const void* uk, void* gk, double s) = 0;
+  virtual int subgrid(const float* net_in, float* net_out, int r, const void* base, const void* fine, void* out,
+                      float s) = 0;
+  virtual int frozen_rk4(double* rows, int64_t np, const void* u, double h, int nsteps) = 0;
 };
 
 #define LD_DECL_FACTORY(TN, NN) EngineBase* make_engine_##TN##_##NN(const ld_params& p);
