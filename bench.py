@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64 x 64^3 ensemble timing")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-against-FP64 accuracy run")
+    ap.add_argument("--no-collisions", action="store_true", help="skip the collision-pair timing")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-subgrid-form comparison")
     ap.add_argument("--no-cnn", action="store_true", help="skip the learned-subgrid CNN timing")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default y-pass block shape")
     args = ap.parse_args()
@@ -381,11 +383,26 @@ def main():
         out["fp32_accuracy"] = rep
         log(f"[bench] spectrum error max {rep['spectrum_err_max']:.2e} (shell {rep['spectrum_err_max_shell']}), "
             f"dissipation error {rep['dissipation_err']:.2e}")
-    if cfg.n_particles:
+    if cfg.n_particles and not args.no_collisions:
         log("[bench] collision pairs and pair statistics")
         out["collisions"] = time_collisions(cfg, coord)
         log(f"[bench] collisions {out['collisions']['ms_collision_pairs']:.2f} ms, "
             f"{out['collisions']['pairs']} pairs")
+    if not args.no_learned:
+        from lucky_droplets_amd.config import Config as _Config
+        from lucky_droplets_amd.learned import collision_comparison
+        log("[bench] learned subgrid form: 256^3 fine DNS vs 128^3 coarse (plain, learned), 10^6 droplets St 0.5")
+        cf = _Config(N=256, precision="f32", nu=2.5e-3, nshells=2, profile="k4")
+        lr = collision_comparison(cf, npart=1_000_000, st=0.5, spinup=500, train_states=12, train_every=10,
+                                  train_iters=1000, relax=300, samples=5, sample_every=20, rmax=0.05, nbins=16)
+        forms = lr.pop("forms")
+        lr["K_sum"] = {k: float(forms[k]["K"].sum()) for k in forms}
+        lr["workload"] = ("fine DNS 256^3 FP32 (nu 2.5e-3) against a 128^3 coarse DNS from the truncated state, "
+                          "droplets one-way coupled in the frozen field of each step; pair statistics up to r = 0.05 "
+                          "over 5 snapshots after 300 steps")
+        out["learned"] = lr
+        log(f"[bench] learned: kernel error plain {lr['kernel_err_plain']:.3f}, learned {lr['kernel_err_learned']:.4f}; "
+            f"ms/step fine {lr['ms_per_step']['fine']:.2f}, learned {lr['ms_per_step']['learned']:.2f}")
     if not args.no_ensemble:
         from lucky_droplets_amd.config import preset as _preset
         c3 = _preset("c3")
