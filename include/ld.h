@@ -151,11 +151,11 @@ int ld_tuning(ld_ctx* ctx, int32_t* ylines, double* us);
 
 /* Every launch-geometry choice (host arrays): choice[6] = y-pass lines per
  * block, z-tile height of the u x omega pass and of the v pass, lines per
- * block of the u/omega, RHS and density x passes; us[13] = the y-pass
+ * block of the u/omega, RHS and density x passes; us[18] = the y-pass
  * candidates' median times (as ld_tuning), then per tuned kernel the times of
- * its default and its alternative (half the height or lines; 0 where not
- * timed).  Every choice gives the same results.  No counterpart in the
- * reference. */
+ * its default, its half (height or lines) and its double (z tiles only)
+ * variant (0 where not timed).  Every choice gives the same results.  No
+ * counterpart in the reference. */
 int ld_tuning_all(ld_ctx* ctx, int32_t* choice, double* us);
 
 /* ---------------------------------------------------------- measurement */

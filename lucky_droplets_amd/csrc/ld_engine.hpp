@@ -74,12 +74,18 @@ struct Engine final : EngineBase {
   // x-pass lines per block (>= 2); a block's lines do not change any line's
   // arithmetic, so every choice gives the same bits
   static constexpr int P6H = P6 / 2 >= 2 ? P6 / 2 : P6, P3H = P3 / 2 >= 2 ? P3 / 2 : P3;
+  // and twice the z-tile height where the block stays within 512 threads and 64 KB of LDS
+  static constexpr bool zfits(int P, int F) {
+    return F * (P / 2) * N1 <= 512 && (long)F * (P / 2) * 2 * Nf * (long)sizeof(C<T>) <= 65536 && P <= N * N;
+  }
+  static constexpr int P6D = zfits(2 * P6, 6) ? 2 * P6 : P6, P3D = zfits(2 * P3, 3) ? 2 * P3 : P3;
   static constexpr int M3H = M3 / 2 >= 2 ? M3 / 2 : M3, M4H = M4 / 2 >= 2 ? M4 / 2 : M4;
-  // launch-geometry choices: 0 = default, 1 = the alternative above, for the
+  // launch-geometry choices: 0 = default, 1 = the alternative above (2 = the
+  // doubled z tile, z passes only), for the
   // u x omega z pass, the v z pass, the u/omega x pass, the RHS x pass and the
   // density x pass (set by the autotuner or LD_TUNE_FORCE=a,b,c,d,e)
   int vsel[5] = {0, 0, 0, 0, 0};
-  double vus[5][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  double vus[5][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   static constexpr int NBLK_SUM = 1024;             // blocks of k_sum2
 
   ld_params p;
@@ -228,14 +234,15 @@ struct Engine final : EngineBase {
     LD_TRY(fill<C<T>>(st, acc, 3 * CF, C<T>{}));
     LD_TRY(fill<T>(st, n_state, RF, (T)0));
     const bool alt[5] = {P6H != P6, P3H != P3, M3H != M3, M3H != M3, M4H != M4};
+    const bool dbl[5] = {P6D != P6, P3D != P3, false, false, false};
     RhsArgs<T> ra;
     ra.nl = w6; ra.us = us; ra.ubase = uk; ra.acc = acc; ra.usnext = us; ra.ku = nullptr;
     ra.fac = fac; ra.fs = CF; ra.nu_eff = (T)(p.nu * p.isexplicit); ra.tps = (T)p.tps;
     ra.wacc = (T)0; ra.anext = (T)0; ra.scale = inv(); ra.acc_init = 0; ra.kcut = kc; ra.band = kc < N / 2 ? 1 : 0;
     for (int k = 0; k < 5; ++k) {
-      for (int v = 0; v < 2; ++v) {
+      for (int v = 0; v < 3; ++v) {
         vus[k][v] = 0.0;
-        if (v == 1 && !alt[k]) continue;
+        if ((v == 1 && !alt[k]) || (v == 2 && !dbl[k])) continue;
         vsel[k] = v;
         auto launch = [&]() -> int {
           switch (k) {
@@ -248,7 +255,10 @@ struct Engine final : EngineBase {
         };
         LD_TRY(time_us(e0, e1, launch, vus[k][v]));
       }
-      vsel[k] = (alt[k] && vus[k][1] > 0.0 && vus[k][1] < vus[k][0]) ? 1 : 0;
+      int best = 0;
+      for (int v = 1; v < 3; ++v)
+        if (vus[k][v] > 0.0 && vus[k][v] < vus[k][best]) best = v;
+      vsel[k] = best;
     }
     return LD_OK;
   }
@@ -285,14 +295,15 @@ struct Engine final : EngineBase {
     return rc;
   }
   int tuning_all(int32_t* choice, double* us_out) override {
-    const int32_t sizes[5][2] = {{P6, P6H}, {P3, P3H}, {M3, M3H}, {M3, M3H}, {M4, M4H}};
+    const int32_t sizes[5][3] = {{P6, P6H, P6D}, {P3, P3H, P3D}, {M3, M3H, M3}, {M3, M3H, M3}, {M4, M4H, M4}};
     if (choice) {
       choice[0] = my;
       for (int k = 0; k < 5; ++k) choice[1 + k] = sizes[k][vsel[k]];
     }
     if (us_out) {
       for (int c = 0; c < 3; ++c) us_out[c] = tune_us[c];
-      for (int k = 0; k < 5; ++k) { us_out[3 + 2 * k] = vus[k][0]; us_out[4 + 2 * k] = vus[k][1]; }
+      for (int k = 0; k < 5; ++k)
+        for (int v = 0; v < 3; ++v) us_out[3 + 3 * k + v] = vus[k][v];
     }
     return LD_OK;
   }
@@ -414,7 +425,7 @@ struct Engine final : EngineBase {
     // LD_TUNE_FORCE=a,b,c,d,e (0/1 each): force the geometry choices (measurement, tests)
     if (const char* e = getenv("LD_TUNE_FORCE")) {
       for (int k = 0; k < 5 && *e; ++k) {
-        vsel[k] = (*e == '1') ? 1 : 0;
+        vsel[k] = (*e == '1') ? 1 : ((*e == '2' && k < 2) ? 2 : 0);
         while (*e && *e != ',') ++e;
         if (*e == ',') ++e;
       }
@@ -591,6 +602,10 @@ struct Engine final : EngineBase {
   }
   // kcu < N/2: the stage u_hat has no nonzero mode outside the mask
   int zcross(hipStream_t s, T* uphys, V4<T>* g, int x0, int nx) {
+    if (vsel[0] == 2) {
+      OpZCross<T, N, P6D> oc{w6, CF, uphys, g, inv()};
+      return zrun<P6D, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc);
+    }
     if (vsel[0]) {
       OpZCross<T, N, P6H> oc{w6, CF, uphys, g, inv()};
       return zrun<P6H, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc);
@@ -612,6 +627,10 @@ struct Engine final : EngineBase {
   // v_hat y-inverse and z pass (v out, n v forward) slab by slab; `wait`
   // (may be null) is waited for before the first z pass writes v / nv.
   int zvel(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip, int x0, int nx) {
+    if (vsel[1] == 2) {
+      OpZVel<T, N, P3D> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
+      return zrun<P3D, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc);
+    }
     if (vsel[1]) {
       OpZVel<T, N, P3H> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
       return zrun<P3H, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc);

@@ -263,12 +263,12 @@ class ForcedDNS:
     def tuning_all(self):
         """Every launch-geometry choice: (dict of choices, dict of timings in us)."""
         ch = (ctypes.c_int32 * 6)()
-        us = (ctypes.c_double * 13)()
+        us = (ctypes.c_double * 18)()
         _lib.check(self.lib.ld_tuning_all(self.ctx, ch, us))
         names = ["y_lines", "z_cross_tile", "z_vel_tile", "x_uw_lines", "x_rhs_lines", "x_kn_lines"]
         times = {"y_lines": list(us[0:3])}
         for k, nm in enumerate(names[1:]):
-            times[nm] = [us[3 + 2 * k], us[4 + 2 * k]]
+            times[nm] = [us[3 + 3 * k], us[4 + 3 * k], us[5 + 3 * k]]
         return dict(zip(names, ch[:])), times
 
     def status_all(self):
