@@ -12,7 +12,6 @@ Follows ``forced-dns-sm.py`` with one MPI rank (``Np == N``):
   full_RHS                  forced-dns-sm.py:287-332
   RK4                       forced-dns-sm.py:335-362
   step epilogue             forced-dns-sm.py:582-620
-  initial condition         forced-dns-sm.py:691-737 and :567-569
 
 Layouts (C order): physical ``(3, N, N, N)`` as ``[c][x][y][z]``; spectral
 ``(3, N, N, N//2+1)`` as ``[c][kx][ky][kz]``.  Forward transforms are
@@ -277,47 +276,3 @@ class Flow:
         if uk.max() > 100 * self.N ** 3:                                 # :618
             raise BlowUp("Threshold exceeded")
         return uk, n
-
-    # -------------------------------------------------- initial condition
-    def initial_condition(self, rng, einit=None, profile="reference", kinit=31):
-        """Random-phase solenoidal initial field (forced-dns-sm.py:691-737, :567-569).
-
-        ``profile="reference"`` is the reference's |u_k|^2 ~ exp(-kint^2/2)/normalize
-        (:700-703); ``profile="k4"`` maps E(k) ~ k^4 exp(-2 (k/3)^2) the same way
-        (eprofile = kint^4 exp(-2 (kint/3)^2) / normalize).  Phases are drawn from
-        ``rng.uniform(0, 2 pi, (N, N, Nf))`` in the order u, v, w (:695-697).
-        Returns ``(uk, n)`` as the loop sees them after ``uk = rfft(u) * dealias``.
-        """
-        G = self.grid
-        N = self.N
-        TWO_PI = 2 * np.pi
-        if einit is None:
-            einit = TWO_PI ** 3                                          # :86
-        thu = rng.uniform(0, TWO_PI, G.k.shape)                          # :695-697
-        thv = rng.uniform(0, TWO_PI, G.k.shape)
-        thw = rng.uniform(0, TWO_PI, G.k.shape)
-        kint = G.kint
-        if profile == "reference":
-            eprofile = kint ** 2 * np.exp(-kint ** 2 / 2) / G.normalize  # :700
-        elif profile == "k4":
-            eprofile = kint ** 4.0 * np.exp(-2.0 * (kint / 3.0) ** 2) / G.normalize
-        else:
-            raise ValueError(profile)
-        amp = (eprofile / np.where(kint == 0, np.inf, kint ** 2)) ** 0.5  # :703
-        sel = (kint ** 2 < kinit ** 2) * (kint > 0) * G.dealias
-        uk = np.empty((3, N, N, G.Nf), dtype=np.complex128)
-        uk[0] = amp * np.exp(1j * thu) * sel                             # :705-707
-        uk[1] = amp * np.exp(1j * thv) * sel
-        uk[2] = amp * np.exp(1j * thw) * sel
-        u = np.stack([G.irfft_mpi(uk[c]) for c in range(3)])             # :709-711
-        uk = np.stack([G.rfft_mpi(u[c]) for c in range(3)])              # :713-715
-        trm = G.kx * uk[0] + G.ky * uk[1] + G.kz * uk[2]                 # :717
-        uk[0] = uk[0] + G.invlap * G.kx * trm                            # :718-720
-        uk[1] = uk[1] + G.invlap * G.ky * trm
-        uk[2] = uk[2] + G.invlap * G.kz * trm
-        e0 = np.sum(G.e3d_to_e1d(G.energy3d(uk)))                        # :722-725
-        uk = uk * (einit / e0) ** 0.5                                    # :726-728
-        u = np.stack([G.irfft_mpi(uk[c]) for c in range(3)])             # :731-733
-        n = np.ones((N, N, N))                                           # :736
-        uk = np.stack([G.rfft_mpi(u[c]) * G.dealias for c in range(3)])  # :567-569
-        return uk, n, u

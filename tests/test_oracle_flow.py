@@ -325,15 +325,33 @@ def test_step_projection_and_blowup():
 
 
 # ------------------------------------------------------------------ initial condition
-def test_ic_product_equals_oracle_recipe():
-    cfg = Config(N=16, nu=0.02, nshells=2, profile="k4")
-    fl = make_flow_oracle(cfg)
-    a = ic.initial_flow(cfg, np.random.default_rng(11))
-    b = fl.initial_condition(np.random.default_rng(11), profile="k4")
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
-    G = fl.grid
-    div = G.kx * a[0][0] + G.ky * a[0][1] + G.kz * a[0][2]
-    assert np.max(np.abs(div)) < 1e-9 * np.max(np.abs(a[0]))
+def test_initial_flow_properties():
+    """The host-side initial condition (lucky_droplets_amd.ic, the recipe of
+    forced-dns-sm.py:691-737 and :567-569), checked by properties the recipe
+    fixes: solenoidal, energy sum_k E(k) = (2 pi)^3, no mode at k = 0, outside
+    the dealiasing mask or at kint >= kinit, Hermitian kz = 0 and N/2 planes,
+    n = 1, and the same field for the same seed."""
+    for prof in ("k4", "reference"):
+        cfg = Config(N=16, nu=0.02, nshells=2, profile=prof, kinit=6)
+        fl = make_flow_oracle(cfg)
+        G = fl.grid
+        uk, n, u = ic.initial_flow(cfg, np.random.default_rng(11))
+        div = G.kx * uk[0] + G.ky * uk[1] + G.kz * uk[2]
+        assert np.max(np.abs(div)) < 1e-12 * np.max(np.abs(uk))
+        E = G.e3d_to_e1d(G.energy3d(uk))
+        assert abs(E.sum() - (2 * np.pi) ** 3) < 1e-10 * (2 * np.pi) ** 3
+        outside = (~G.dealias) | (G.kint == 0) | (G.kint >= cfg.kinit)
+        assert np.max(np.abs(uk[:, outside])) < 1e-12 * np.max(np.abs(uk))
+        assert np.max(np.abs(uk[:, ~outside])) > 0
+        for iz in (0, 8):
+            P = uk[:, :, :, iz]
+            Q = np.conj(np.roll(np.flip(P, axis=(1, 2)), 1, axis=(1, 2)))
+            assert np.max(np.abs(P - Q)) < 1e-12 * np.max(np.abs(uk))
+        assert np.array_equal(n, np.ones((16, 16, 16)))
+        uk2, _, _ = ic.initial_flow(cfg, np.random.default_rng(11))
+        assert np.array_equal(uk, uk2)
+        # u is the physical field the loop transforms (uk = rfft(u) * dealias)
+        assert relerr(np.stack([G.rfft_mpi(u[c]) for c in range(3)]) * G.dealias, uk) < 1e-14
 
 
 def test_divergence_closed_forms():
