@@ -299,7 +299,10 @@ int ld_cnn_train_step(ld_cnn* c, const float* x, const float* target, float lr, 
  * ld_droplets_frozen_rk4: nsteps RK4 steps of size h of droplet rows (np, 7)
  *   in place in the frozen physical velocity u (3 fields): x' = v,
  *   v' = (I_u(x) - v) / (St tau_eta) - g e_z, I_u the cosine-kernel gather
- *   (particles.py:435-511), St = (m / factor)^(2/3) (particles.py:823), m fixed. */
+ *   (particles.py:435-511), St = (m / factor)^(2/3) (particles.py:823), m fixed;
+ *   np < 2^29.  Uses the context's droplet gather records as scratch (not
+ *   to be called while an ld_advance of the same context is in flight on
+ *   another stream). */
 int ld_spectral_resample(ld_ctx* ctx, const void* src, int32_t n_src, void* dst, int32_t nfields);
 int ld_velocity_gradient_spec(ld_ctx* ctx, const void* uk, void* gk, double scale);
 int ld_fields_to_rows(ld_ctx* ctx, const void* f, int32_t nfields, float* rows, float scale);

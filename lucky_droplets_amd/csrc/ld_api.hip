@@ -405,7 +405,7 @@ int ld_droplets_frozen_rk4(ld_ctx* ctx, double* rows, int64_t np, const void* u,
   LD_CHECK_CTX(ctx); LD_ONE(ctx); LD_FIELD(u);
   if (np < 0 || nsteps < 0) { set_error("np and nsteps must be >= 0"); return LD_ERR_ARG; }
   if (np > 0) LD_NONNULL(rows);
-  if (np > 0x7fffffff) { set_error("np too large"); return LD_ERR_ARG; }
+  if (np > 0x1fffffff) { set_error("np too large (at most 2^29 - 1)"); return LD_ERR_ARG; }
   return ctx->eng->frozen_rk4(rows, np, u, h, nsteps);
 }
 

@@ -1493,8 +1493,14 @@ struct Engine final : EngineBase {
   int frozen_rk4(double* rows, int64_t n_, const void* u, double h, int nsteps) override {
     if (n_ <= 0 || nsteps <= 0) return LD_OK;
     LD_TRY(join(st, st2, evB));
-    hipLaunchKernelGGL((k_frozen_rk4<T, N>), dim3((unsigned)((n_ + 255) / 256)), dim3(256), 0, st, rows, (int)n_,
-                       (const T*)u, h, nsteps, dp);
+    // the step's gather records are free between steps
+    V4<T>* g = grec[0];
+    hipLaunchKernelGGL((k_u_records<T, N>), dim3(2048), dim3(256), 0, st, (const T*)u, g);
+    LD_LAUNCHED();
+    // 4 lanes per droplet; block count a multiple of 8 for the XCD mapping
+    const unsigned nb = (unsigned)(((4 * n_ + 255) / 256 + 7) / 8 * 8);
+    hipLaunchKernelGGL((k_frozen_rk4<T, N>), dim3(nb), dim3(256), 0, st, rows, (int)n_, (const V4<T>*)g, h, nsteps,
+                       dp);
     LD_LAUNCHED();
     return LD_OK;
   }

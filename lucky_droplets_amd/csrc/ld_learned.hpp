@@ -92,46 +92,106 @@ k_subgrid(const float* __restrict__ net_in, float* __restrict__ net_out, int r, 
   }
 }
 
-// Droplets advanced nsteps classical RK4 steps of size h in a frozen velocity
-// field u (3 physical fields on this grid): x' = v, v' = (I_u(x) - v) / (St
-// tau_eta) - g e_z with I_u the cosine-kernel gather (particles.py:435-511) and
-// St = (m / factor)^(2/3) (particles.py:823); the mass does not change.  Rows
-// (Np, 7) float64 in place, positions unwrapped.  One thread per droplet.
+// The velocity u (3 physical fields) as the droplet gather's u records
+// {u_x, u_y, u_z, 0} (recU: z-quads of four 16-byte records, so the four z
+// nodes of a stencil column span one or two 128-byte lines).
 template <typename T, int N>
 __global__ void __launch_bounds__(256)
-k_frozen_rk4(double* __restrict__ rows, int np, const T* __restrict__ u, double h, int nsteps, DropPhys P) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
+k_u_records(const T* __restrict__ u, V4<T>* __restrict__ g) {
   constexpr size_t R = (size_t)N * N * N;
-  double* row = rows + (size_t)p * 7;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < R; o += (size_t)gridDim.x * blockDim.x) {
+    V4<T> q;
+    q.x = u[o]; q.y = u[R + o]; q.z = u[2 * R + o]; q.w = (T)0;
+    g[recU((uint32_t)o)] = q;
+  }
+}
+
+// Droplets advanced nsteps classical RK4 steps of size h in a frozen velocity
+// field (its records g from k_u_records): x' = v, v' = (I_u(x) - v) / (St
+// tau_eta) - g e_z with I_u the cosine-kernel gather (particles.py:435-511) and
+// St = (m / factor)^(2/3) (particles.py:823); the mass does not change.  Rows
+// (Np, 7) float64 in place, positions unwrapped.  Four lanes per droplet, lane
+// c owning z node c of the 4x4x4 stencil, as in k_particles_stage (the same
+// wrap, stencil origin and weights: sines and cosines and the gather sums in
+// the field precision, positions and the RK update in FP64).  Blocks map
+// XCD-aware: each XCD takes one contiguous range of rows, so rows sorted by
+// position share its L2.
+template <typename T, int N>
+__global__ void __launch_bounds__(256)
+k_frozen_rk4(double* __restrict__ rows, int np, const V4<T>* __restrict__ g, double h, int nsteps, DropPhys P) {
+  const int nb = gridDim.x, per = nb / 8;
+  const int lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int gt = lb * blockDim.x + threadIdx.x;
+  const int p = gt >> 2, c = gt & 3;
+  const bool live = p < np;
+  double* row = rows + (size_t)(live ? p : 0) * 7;
+  using GT = std::conditional_t<sizeof(T) == 4, float, double>;
   double x[3], v[3];
+#pragma unroll
   for (int k = 0; k < 3; ++k) { x[k] = row[k]; v[k] = row[3 + k]; }
   const double st = pow(row[6] / P.factor, 2.0 / 3.0);
   const double rden = 1.0 / (st * P.tau_eta);
-  const double a[4] = {0.0, 0.5, 0.5, 1.0}, w[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  const double a_[4] = {0.0, 0.5, 0.5, 1.0}, w_[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  const GT q = (GT)0.25;
   for (int it = 0; it < nsteps; ++it) {
     double kx[3] = {0, 0, 0}, kv[3] = {0, 0, 0}, sx[3], sv[3];
+#pragma unroll
     for (int k = 0; k < 3; ++k) { sx[k] = x[k]; sv[k] = v[k]; }
+#pragma unroll 1
     for (int s = 0; s < 4; ++s) {
       double xs[3], vs[3], xw[3];
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
-        xs[k] = x[k] + a[s] * h * kx[k];
-        vs[k] = v[k] + a[s] * h * kv[k];
-        xw[k] = wrapL(xs[k], P.L);
+        xs[k] = x[k] + a_[s] * h * kx[k];
+        vs[k] = v[k] + a_[s] * h * kv[k];
+        xw[k] = wrapFast(xs[k], P.L, P.invL);
       }
-      Stencil<N> sten;
-      sten.set(xw, P.dx, P.pi_2dx);
-      double I[3];
-      gatherN<T, N, 3>(sten, u, R, I);
+      const int i0 = originOf(xw[0], P.inv_dx), j0 = originOf(xw[1], P.inv_dx), k0 = originOf(xw[2], P.inv_dx);
+      GT xsn, xcs, ysn, ycs, zsn, zcs;
+      sincosGT((xw[0] - (double)i0 * P.dx) * P.pi_2dx, xsn, xcs);
+      sincosGT((xw[1] - (double)j0 * P.dx) * P.pi_2dx, ysn, ycs);
+      sincosGT((xw[2] - (double)k0 * P.dx) * P.pi_2dx, zsn, zcs);
+      const GT wy[4] = {q * ((GT)1 + ycs), q * ((GT)1 + ysn), q * ((GT)1 - ycs), q * ((GT)1 - ysn)};
+      const GT wz = q * ((GT)1 + ((c & 2) ? (GT)-1 : (GT)1) * ((c & 1) ? zsn : zcs));
+      const uint32_t iz = (uint32_t)wrapIdx<N>(k0 + c);
+      GT Ig[3] = {(GT)0, (GT)0, (GT)0};
+      if (live) {
+        V4<T> U[16];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const uint32_t plane = (uint32_t)wrapIdx<N>(i0 + a) * N;
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            U[a * 4 + b] = ldRec<T, N>(g, recU((plane + (uint32_t)wrapIdx<N>(j0 + b)) * N + iz));
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const GT wx = q * ((GT)1 + ((a & 2) ? (GT)-1 : (GT)1) * ((a & 1) ? xsn : xcs));
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const GT w = wy[b] * wx;
+            Ig[0] += (GT)U[a * 4 + b].x * w;
+            Ig[1] += (GT)U[a * 4 + b].y * w;
+            Ig[2] += (GT)U[a * 4 + b].z * w;
+          }
+        }
+      }
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
+        GT t = Ig[k] * wz;
+        t += __shfl_xor(t, 1, 64);
+        t += __shfl_xor(t, 2, 64);
         kx[k] = vs[k];
-        kv[k] = (I[k] - vs[k]) * rden - (k == 2 ? P.g : 0.0);
-        sx[k] += w[s] * h * kx[k];
-        sv[k] += w[s] * h * kv[k];
+        kv[k] = ((double)t - vs[k]) * rden - (k == 2 ? P.g : 0.0);
+        sx[k] += w_[s] * h * kx[k];
+        sv[k] += w_[s] * h * kv[k];
       }
     }
+#pragma unroll
     for (int k = 0; k < 3; ++k) { x[k] = sx[k]; v[k] = sv[k]; }
   }
+  if (!live || c != 0) return;
+#pragma unroll
   for (int k = 0; k < 3; ++k) { row[k] = x[k]; row[3 + k] = v[k]; }
 }
 

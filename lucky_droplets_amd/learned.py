@@ -54,7 +54,8 @@ class SubgridModel:
         self.lib = self.fine.lib
         self.channels = (12,) + tuple(hidden) + (3 * self.r ** 3,)
         self.net = SubgridCNN(self.Nc, self.channels, device=device)
-        self.net.init_params(seed=seed, gain=0.1)
+        # last layer zero: training starts from the plain reconstruction (base)
+        self.net.init_params(seed=seed, gain=0.0)
         self.sx = self.sg = self.sy = 1.0
         self._ukf = self.fine.zeros_spec(3)
         self._base = self.fine.zeros_real(3)
@@ -129,7 +130,12 @@ class SubgridModel:
         self.sg = math.sqrt(gs / len(uk_fines))
         self.sy = math.sqrt(ts / len(uk_fines))
 
-    def train(self, pairs, iters: int, lr: float = 3e-3, lr_end: float = 3e-4) -> List[float]:
+    def train(self, pairs, iters: int, lr: float = 3e-4, lr_end: float = 3e-5) -> List[float]:
+        """Adam steps over the pairs in turn, the step size decaying
+        geometrically from lr to lr_end.  The first Adam steps move every
+        weight by about the step size; at 3e-3 that switched off every hidden
+        ReLU of this network (fan-in 27 x 32) for these targets, which left it
+        at the plain reconstruction."""
         losses = []
         for it in range(iters):
             x, t = pairs[it % len(pairs)]

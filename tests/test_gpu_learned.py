@@ -126,9 +126,9 @@ def test_learned_form_collision_kernel_against_fine_dns(gpu):
     """128^3 fine DNS against the 64^3 coarse DNS with plain interpolation and
     with the learned reconstruction, 2 x 10^5 droplets (St 0.5): the learned
     form's collision kernel (summed over the separation bins up to 0.1) is
-    within 5 % of the fine DNS's and closer than the plain coarse form's; the
-    trained network's subgrid velocity is no worse than none on a held-out
-    state."""
+    within 5 % of the fine DNS's and closer than the plain coarse form's; on a
+    held-out state the trained network's subgrid velocity has at least 5 %
+    less mean squared error than none (the plain reconstruction)."""
     from lucky_droplets_amd.learned import collision_comparison
     cfg = Config(N=128, precision="f32", nu=0.0093, nshells=2, profile="k4")
     r = collision_comparison(cfg, npart=200_000, st=0.5, spinup=300, train_states=8, train_every=10,
@@ -139,4 +139,4 @@ def test_learned_form_collision_kernel_against_fine_dns(gpu):
         assert np.all(np.isfinite(f["K"])) and np.all(f["counts"] > 0)
     assert r["kernel_err_learned"] < 0.05
     assert r["kernel_err_learned"] < r["kernel_err_plain"]
-    assert r["heldout_mse_learned"] <= 1.01 * r["heldout_mse_plain"]
+    assert r["heldout_mse_learned"] < 0.95 * r["heldout_mse_plain"]
