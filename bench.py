@@ -56,12 +56,12 @@ def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None, graph=Fa
     """ms per step over `steps` timed steps (after `warmup`), plus the summed
     HIP-event duration and launch count of `profile_kernel` inside the timed region
     (profiling runs the steps eagerly; graph=True replays a captured HIP graph;
-    autotune=True lets the context pick the y-pass block shape at creation)."""
+    autotune=True lets the context time and pick its launch geometries at creation)."""
     import torch
     from lucky_droplets_amd.dns import ForcedDNS
     dev = torch.device("cuda:0")
     d = ForcedDNS(cfg, graph=graph, autotune=autotune)
-    tune = d.tuning() if autotune else None
+    tune = d.tuning_all() if autotune else None
     ukg = torch.from_numpy(uk).to(dev).to(d.cdtype)
     ng = torch.from_numpy(n).to(dev).to(d.rdtype)
     cg = torch.from_numpy(coord).to(dev)
@@ -312,7 +312,10 @@ def main():
                                                     autotune=tuned)
     tuning = time_gpu.last_tuning
     log(f"[bench] FP32 {ms32:.3f} ms/step; droplet kernel {kms / max(klaunch, 1):.3f} ms x {klaunch}"
-        + (f"; y-pass lines per block {tuning[0]} (us {tuning[1]})" if tuning else ""))
+        + (f"; launch choices {tuning[0]}" if tuning else ""))
+    if tuning:
+        log("[bench] autotuner median us per candidate: "
+            + ", ".join(f"{k} {[round(x, 1) for x in v]}" for k, v in tuning[1].items()))
     ms_eager = ms32
     msg = None
     if not args.no_graph:
@@ -331,7 +334,7 @@ def main():
         "ms_per_step": ms32,
         "eager_ms_per_step": ms_eager,
         "graph_ms_per_step": msg,
-        "y_pass_tuning": ({"lines_per_block": tuning[0], "median_us_M_Mhalf_2M": tuning[1]} if tuning else None),
+        "launch_tuning": ({"choices": tuning[0], "median_us_per_candidate": tuning[1]} if tuning else None),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -602,6 +602,19 @@ template <typename T> struct RhsArgs {
 };
 
 // ---------------------------------------------- field-parallel x passes
+// Block order of the x passes.  The hardware deals consecutive blocks to the
+// 8 XCDs in turn; here logical blocks 2j and 2j + 1 run on one XCD, one right
+// after the other, and the pairs are dealt in turn.  With 8 complex64 lines
+// per block (64-byte row segments) the two blocks of a pair share every
+// 128-byte row segment in that XCD's L2 instead of fetching it twice, while
+// work stays interleaved over the XCDs (lines outside the dealiasing radius
+// cost less).  Grids that are not a multiple of 16 keep the hardware order.
+__device__ __forceinline__ int xPairBlock() {
+  const int nb = gridDim.x, pb = blockIdx.x;
+  if (nb % 16) return pb;
+  const int x = pb % 8, k = pb / 8;
+  return 2 * (8 * (k >> 1) + x) + (k & 1);
+}
 // The kernels below give each of the three vector components its own thread
 // group (3 * M * N1 threads): every thread transforms one field, so a thread
 // holds N2 elements instead of 3 * N2, and the components meet in LDS for the
@@ -623,7 +636,7 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
   __shared__ C<T> lds[3 * M * (N + 1)];
   const int f = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
-  const int q = blockIdx.x * M + m;
+  const int q = xPairBlock() * M + m;
   const T ky = (T)kwave(q / Nf, N), kz = (T)(q % Nf);
   C<T>* L[3] = {lds + m * (N + 1), lds + (M + m) * (N + 1), lds + (2 * M + m) * (N + 1)};
   C<T>* Lf = lds + (f * M + m) * (N + 1);
@@ -710,7 +723,7 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
   __shared__ C<T> lds[3 * M * (N + 1)];
   const int f = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
-  const int q = blockIdx.x * M + m;
+  const int q = xPairBlock() * M + m;
   const int iy = q / Nf, iz = q % Nf;
   C<T>* L0 = lds + m * (N + 1);
   C<T>* L1 = lds + (M + m) * (N + 1);
@@ -897,7 +910,7 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs, in
   __shared__ C<T> lds[4 * M * (N + 1)];
   const int g = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
-  const int q = blockIdx.x * M + m;
+  const int q = xPairBlock() * M + m;
   const int iy = q / Nf, iz = q % Nf;
   C<T>* L0 = lds + m * (N + 1);
   C<T>* L1 = lds + (M + m) * (N + 1);
