@@ -322,11 +322,10 @@ k_particles_stage(PStage<T, PT> A0) {
   const int pp = live ? p : 0;
   constexpr size_t R = (size_t)N * N * N;
   const int cap = A.cap;
-  // Loads are grouped into three round trips (stage position; the gather;
+  // Loads are grouped into two round trips (stage position; the gather with
   // velocity, mass and the sump inputs), with uniform pointer selects rather
   // than branches between a load and its use: stage 0 reads x0 + 0 * x0 = x0
-  // (exact for finite values).  The stage velocity and mass are read after the
-  // gather, so they hold no registers while its loads are in flight.
+  // (exact for finite values).
   const bool s0 = A.stage == 0;
   const double ah = s0 ? 0.0 : A.ah;
   const double* kxp = s0 ? A.x0 : A.kx;
@@ -375,6 +374,19 @@ k_particles_stage(PStage<T, PT> A0) {
   const GT wz = q * ((GT)1 + ((c & 2) ? (GT)-1 : (GT)1) * ((c & 1) ? zsn : zcs));
   const GT gxs = xsn, gxc = xcs;
   const int iz = wrapIdx<N>(k0 + c);
+  // stage velocity and mass (v0 + a h kv) and the sump inputs (x0, v0 at
+  // stage 0, else the accumulators), issued before the gather so that their
+  // round trip overlaps its loads (measured 0.320 -> 0.306 ms per launch at
+  // 256^3 with 10^6 droplets, though occupancy drops from 5 to 4 waves/SIMD)
+  const PT* kvp = s0 ? A.v0 : A.kv;
+  const double* sxp = s0 ? A.x0 : A.sx;
+  const PT* svp = s0 ? A.v0 : A.sv;
+  PT v0v[4], kvv[4], prv[4];
+  double prx[3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { v0v[k] = A.v0[k * cap + pp]; kvv[k] = kvp[k * cap + pp]; prv[k] = svp[k * cap + pp]; }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) prx[k] = sxp[k * cap + pp];
   // gather: this lane's z node, all (x, y) nodes; two 16-byte records per node
   GT Ig[7];
 #pragma unroll
@@ -441,17 +453,6 @@ k_particles_stage(PStage<T, PT> A0) {
     v += __shfl_xor(v, 2, 64);
     I[k] = (double)v;
   }
-  // stage velocity and mass (v0 + a h kv) and the sump inputs (x0, v0 at
-  // stage 0, else the accumulators)
-  const PT* kvp = s0 ? A.v0 : A.kv;
-  const double* sxp = s0 ? A.x0 : A.sx;
-  const PT* svp = s0 ? A.v0 : A.sv;
-  PT v0v[4], kvv[4], prv[4];
-  double prx[3];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) { v0v[k] = A.v0[k * cap + pp]; kvv[k] = kvp[k * cap + pp]; prv[k] = svp[k * cap + pp]; }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) prx[k] = sxp[k * cap + pp];
   double rhs[7], W = 0.0;
   if constexpr (sizeof(PT) == 4) {
     float vpf[3], If[7], rf[7], Wf = 0.0f;
