@@ -134,7 +134,8 @@ struct Engine final : EngineBase {
   struct ArenaSlot { void** ptr; size_t off; };
   std::vector<ArenaSlot> arena_plan;
   size_t arena_bytes = 0;
-  double *x0 = nullptr, *sx = nullptr, *kxp = nullptr;
+  double *x0 = nullptr, *sx = nullptr;
+  PT *kxp = nullptr;                 // stage dx/dt = stage velocity, held exactly in PT
   PT *v0 = nullptr, *sv = nullptr, *kvp = nullptr;
   double *rows_a = nullptr, *rows_b = nullptr, *wrow = nullptr;
   // spatial sort: slot -> input row, bin keys/ranks, bin starts, double buffers

@@ -278,7 +278,8 @@ template <typename T, typename PT> struct PStage {
   int np, cap, stage;          // stage 0..3
   double* x0; PT* v0;          // RK base (unwrapped); final stage writes the new state here
   double* sx; PT* sv;          // sump accumulator
-  double* kx; PT* kv;          // stage rhs (read: previous stage, write: this stage)
+  PT* kx; PT* kv;              // stage rhs (read: previous stage, write: this stage); dx/dt is
+                               // the stage velocity, which PT holds exactly
   const V4<T>* gu; const V4<T>* gv;   // gather records {u, 0} (recU) and {v, n_stage} (recV); one array
   double4* dep;                // out: wrapped stage position and W per slot (brick deposit input)
   int* flist; int* fcount;     // out: slots whose stage bin moved > 1 bin since the sort
@@ -323,17 +324,19 @@ k_particles_stage(PStage<T, PT> A0) {
   constexpr size_t R = (size_t)N * N * N;
   const int cap = A.cap;
   // Loads are grouped into two round trips (stage position; the gather with
-  // velocity, mass and the sump inputs), with uniform pointer selects rather
-  // than branches between a load and its use: stage 0 reads x0 + 0 * x0 = x0
-  // (exact for finite values).
+  // velocity, mass and the sump inputs), with uniform pointer selects or
+  // branches (stage 0 takes k = 0 and the sump inputs from the RK base).
   const bool s0 = A.stage == 0;
   const double ah = s0 ? 0.0 : A.ah;
-  const double* kxp = s0 ? A.x0 : A.kx;
-  double x0v[3], xs[3];
+  double x0v[3], xs[3], kxv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < 3; ++k) x0v[k] = A.x0[k * cap + pp];
+  if (!s0) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) xs[k] = x0v[k] + ah * kxp[k * cap + pp];
+    for (int k = 0; k < 3; ++k) kxv[k] = (double)A.kx[k * cap + pp];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) xs[k] = x0v[k] + ah * kxv[k];
   const double L = A.P.L, dx = A.P.dx, pi_2dx = A.P.pi_2dx, invL = A.P.invL, inv_dx = A.P.inv_dx;
   const double xw = wrapFast(xs[0], L, invL), yw = wrapFast(xs[1], L, invL), zw = wrapFast(xs[2], L, invL);
   // one sincos per axis (cosW4); node i0 + a has weight (1 +- {cos, sin})/4.
@@ -499,7 +502,7 @@ k_particles_stage(PStage<T, PT> A0) {
   for (int k = 0; k < 4; ++k) dvp[k * cap + p] = prv[k] + (PT)A.wh * (PT)rhs[3 + k];
   if (A.stage < 3) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) A.kx[k * cap + p] = rhs[k];
+    for (int k = 0; k < 3; ++k) A.kx[k * cap + p] = (PT)rhs[k];
 #pragma unroll
     for (int k = 0; k < 4; ++k) A.kv[k * cap + p] = (PT)rhs[3 + k];
   }
