@@ -52,7 +52,8 @@ def make_inputs(cfg):
     return uk, n, coord, pid, dt
 
 
-def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None, graph=False, autotune=False):
+def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None, graph=False, autotune=False,
+             extra_profile=None):
     """ms per step over `steps` timed steps (after `warmup`), plus the summed
     HIP-event duration and launch count of `profile_kernel` inside the timed region
     (profiling runs the steps eagerly; graph=True replays a captured HIP graph;
@@ -82,6 +83,13 @@ def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None, graph=Fa
     kern = d.profile_read() if profile_kernel else (0.0, 0)
     if profile_kernel:
         d.profile(profile_kernel, False)
+    # further kernel classes, each timed over as many steps after the timed region
+    time_gpu.extra = {}
+    for name in extra_profile or ():
+        d.profile(name, True)
+        d.advance(dt, steps)
+        time_gpu.extra[name] = d.profile_read()
+        d.profile(name, False)
     lastmax, blown = d.status()
     d.close()
     time_gpu.last_tuning = tune
@@ -216,10 +224,11 @@ def time_collisions(cfg, coord, reps=5):
 
 
 def droplet_launch_bytes(N, s, npart, sp):
-    """Algorithmic bytes of one droplet-stage launch (one quarter of the
-    per-step droplet terms): min(7R, 448 s Np) + R + 5 Np S_p."""
+    """Algorithmic bytes of one launch of the droplet stage kernel: the
+    per-stage droplet terms of the step without the deposit's write (done by
+    the deposit kernels), min(7R, 448 s Np) + 5 Np S_p."""
     R = N ** 3 * s
-    return min(7 * R, 448 * s * npart) + R + 5 * npart * sp
+    return min(7 * R, 448 * s * npart) + 5 * npart * sp
 
 
 def cpu_baseline(uk, n, coord, cfg, dt, c1_steps=10):
@@ -309,7 +318,9 @@ def main():
     tuned = not args.no_autotune
     ms32, (kms, klaunch), lastmax, blown = time_gpu(cfg, uk, n, coord, dt, args.steps, args.warmup,
                                                     profile_kernel="droplets" if cfg.n_particles else None,
-                                                    autotune=tuned)
+                                                    autotune=tuned,
+                                                    extra_profile=("deposit",) if cfg.n_particles else None)
+    dep_ms, dep_launch = time_gpu.extra.get("deposit", (0.0, 0))
     tuning = time_gpu.last_tuning
     log(f"[bench] FP32 {ms32:.3f} ms/step; droplet kernel {kms / max(klaunch, 1):.3f} ms x {klaunch}"
         + (f"; launch choices {tuning[0]}" if tuning else ""))
@@ -362,6 +373,18 @@ def main():
                            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                            "traffic": traffic, "algorithmic_bytes_per_launch": kb, "avg_launch_ms": avg,
                            "launches": klaunch}
+        if dep_launch:
+            # the deposit (brick pass + fallback) writes the R-byte source field of
+            # each stage; the droplet stage as a whole is both kernels
+            R = cfg.N ** 3 * s
+            davg = dep_ms / dep_launch
+            out["roofline"]["deposit"] = {"kernels": "k_brick_deposit + k_fallback_accum/fold",
+                                          "algorithmic_bytes_per_launch": R, "avg_launch_ms": davg,
+                                          "achieved": R / (davg * 1e-3) / 1e9,
+                                          "frac": R / (davg * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            tot = (kb + R) / ((avg + davg) * 1e-3) / 1e9
+            out["roofline"]["droplet_stage"] = {"algorithmic_bytes_per_stage": kb + R, "ms_per_stage": avg + davg,
+                                                "achieved": tot, "frac": tot / HBM_PEAK_GBS}
     if not args.no_fp64:
         c64 = cfg.replace(precision="f64")
         nsteps64 = max(2, args.steps // 4)

@@ -162,7 +162,7 @@ int ld_tuning_all(ld_ctx* ctx, int32_t* choice, double* us);
 /* Kernel classes that ld_profile can time (HIP events recorded on the stream
  * each launch runs on).  No counterpart in the reference, which only prints
  * wall time per step (forced-dns-sm.py:565-577). */
-#define LD_K_DROPLETS 0   /* fused droplet RK stage: gather, RHS, deposit */
+#define LD_K_DROPLETS 0   /* fused droplet RK stage: gather, RHS, deposit input */
 #define LD_K_XRHS 1       /* x pass: nonlinear term, forcing, projection, RK update, v_hat */
 #define LD_K_XUW 2        /* x pass: u_hat, i k x u_hat */
 #define LD_K_YPASS 3      /* y line passes (all) */
@@ -170,6 +170,7 @@ int ld_tuning_all(ld_ctx* ctx, int32_t* choice, double* us);
 #define LD_K_ZVEL 5       /* z pass: v, n v */
 #define LD_K_XKN 6        /* x pass: density flux */
 #define LD_K_ZKN 7        /* z pass: kn, RK update of n */
+#define LD_K_DEPOSIT 8    /* droplet deposit: brick pass and fallback accumulate/fold */
 /* Start timing every launch of kernel class `kernel` (or stop, enable = 0). */
 int ld_profile(ld_ctx* ctx, int32_t kernel, int32_t enable);
 /* Synchronise, then report the summed duration (ms) and launch count since

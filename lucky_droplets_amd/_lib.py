@@ -14,7 +14,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libld.so"
 
 LD_OK, LD_ERR_ARG, LD_ERR_HIP, LD_ERR_BLOWUP, LD_ERR_CAPACITY = 0, 1, 2, 3, 4
 LD_F32, LD_F64 = 0, 1
-KERNELS = {"droplets": 0, "x_rhs": 1, "x_uw": 2, "y_pass": 3, "z_cross": 4, "z_vel": 5, "x_kn": 6, "z_kn": 7}
+KERNELS = {"droplets": 0, "x_rhs": 1, "x_uw": 2, "y_pass": 3, "z_cross": 4, "z_vel": 5, "x_kn": 6, "z_kn": 7,
+           "deposit": 8}
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 

@@ -264,7 +264,7 @@ int ld_status_all(ld_ctx* ctx, double* last_max, int32_t* blown_up) {
 
 int ld_profile(ld_ctx* ctx, int32_t kernel, int32_t enable) {
   LD_CHECK_CTX(ctx);
-  if (kernel < 0 || kernel > LD_K_ZKN) { set_error("unknown kernel class %d", kernel); return LD_ERR_ARG; }
+  if (kernel < 0 || kernel > LD_K_DEPOSIT) { set_error("unknown kernel class %d", kernel); return LD_ERR_ARG; }
   return ctx->eng->profile(kernel, enable);
 }
 

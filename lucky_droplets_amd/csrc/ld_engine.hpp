@@ -755,6 +755,7 @@ struct Engine final : EngineBase {
         hipLaunchKernelGGL((k_particles_stage<T, PT, N, true>), dim3(nb, 1, B), dim3(256), pstage_lds, st2, ps);
       LD_LAUNCHED();
       LD_TRY(prof_mark(LD_K_DROPLETS, st2));
+      LD_TRY(prof_mark(LD_K_DEPOSIT, st2));
       if (sorted_ok) {
         hipLaunchKernelGGL((k_brick_deposit<T, N>), dim3(Brick<N>::NB * Brick<N>::NB * Brick<N>::NB, 1, B),
                            dim3(Brick<N>::NT), 0, st2, (const double4*)dep, (const int*)hist, (int)np,
@@ -768,6 +769,7 @@ struct Engine final : EngineBase {
       hipLaunchKernelGGL((k_fallback_fold<T, N>), dim3(fb, 1, B), dim3(256), 0, st2, (const double4*)dep,
                          (const int*)flist, (const int*)fcount, (const unsigned long long*)wmaxp, fx, fc, dp, rs);
       LD_LAUNCHED();
+      LD_TRY(prof_mark(LD_K_DEPOSIT, st2));
     }
     OpZKn<T, N, P1> ok;
     ok.w = wn; ok.fs = CF; ok.scale = inv(); ok.kn_out = nullptr;
