@@ -149,13 +149,17 @@ int ld_status_all(ld_ctx* ctx, double* last_max, int32_t* blown_up);
  * counterpart in the reference. */
 int ld_tuning(ld_ctx* ctx, int32_t* ylines, double* us);
 
-/* Every launch-geometry choice (host arrays): choice[6] = y-pass lines per
+/* Every launch-geometry choice (host arrays): choice[7] = y-pass lines per
  * block, z-tile height of the u x omega pass and of the v pass, lines per
- * block of the u/omega, RHS and density x passes; us[18] = the y-pass
- * candidates' median times (as ld_tuning), then per tuned kernel the times of
- * its default, its half (height or lines) and its double (z tiles only)
- * variant (0 where not timed).  Every choice gives the same results.  No
- * counterpart in the reference. */
+ * block of the u/omega, RHS and density x passes, and the threads per line N1
+ * of the y passes' line split N = N1 N2; us[19] = the y-pass candidates'
+ * median times (as ld_tuning), then per tuned kernel the times of its
+ * default, its half (height or lines) and its double (z tiles only) variant
+ * (0 where not timed), then the 6-field y-inverse at the chosen lines with
+ * the other split, N1/2 threads per line.  Every choice the autotuner makes
+ * gives the same results; the split is not one of them (another
+ * factorisation rounds differently): it is timed for the record and set by
+ * the environment (LD_YSPLIT=1).  No counterpart in the reference. */
 int ld_tuning_all(ld_ctx* ctx, int32_t* choice, double* us);
 
 /* ---------------------------------------------------------- measurement */

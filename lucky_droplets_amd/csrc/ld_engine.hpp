@@ -290,6 +290,14 @@ struct Engine final : EngineBase {
     for (int c = 1; c < 3; ++c)
       if (tune_us[c] > 0.0 && tune_us[c] < tune_us[best]) best = c;
     my = cand[best];
+    // the other line split at the chosen lines, timed for the record only
+    {
+      const int keep = ysplit;
+      ysplit = 1;
+      auto launch = [&]() -> int { return pass<1, +1>(st, w6, w6, 6, (T)1); };
+      LD_TRY(time_us(e0, e1, launch, split_us));
+      ysplit = keep;
+    }
     const int rc = autotune_geometry(e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -300,11 +308,13 @@ struct Engine final : EngineBase {
     if (choice) {
       choice[0] = my;
       for (int k = 0; k < 5; ++k) choice[1 + k] = sizes[k][vsel[k]];
+      choice[6] = ysplit ? NQ : N1;
     }
     if (us_out) {
       for (int c = 0; c < 3; ++c) us_out[c] = tune_us[c];
       for (int k = 0; k < 5; ++k)
         for (int v = 0; v < 3; ++v) us_out[3 + 3 * k + v] = vus[k][v];
+      us_out[18] = split_us;
     }
     return LD_OK;
   }
@@ -331,6 +341,7 @@ struct Engine final : EngineBase {
     // LD_SERIAL=1: the flow stream waits for each stage's density chain, so the
     // two chains never run concurrently (per-kernel timing without overlap)
     if (const char* e = getenv("LD_SERIAL")) serial = atoi(e) != 0;
+    if (const char* e = getenv("LD_YSPLIT")) ysplit = atoi(e) != 0;
     if (const char* e = getenv("LD_LATE_DENSITY")) late_density = atoi(e) != 0;
     if (const char* e = getenv("LD_TRUNCATE")) kc = atoi(e) != 0 ? N / 3 : N / 2;
     if (const char* e = getenv("LD_PSTAGE_LDS")) {
@@ -443,6 +454,13 @@ struct Engine final : EngineBase {
   // change the arithmetic.
   static constexpr int MH = M / 2, M2 = 2 * M;
   int my = M;
+  // ysplit = 1 (LD_YSPLIT=1): y passes split each line N = (N1/2) (2 N2)
+  // instead of N1 N2.  Another factorisation rounds differently, so this is
+  // a measurement switch, not an autotuner choice (the autotuner only chooses
+  // between launch shapes that give identical bits); the autotuner times it.
+  static constexpr int NQ = N1 / 2;
+  int ysplit = 0;
+  double split_us = 0.0;
   template <int AXIS, int DIR> int pass(hipStream_t s, const C<T>* in, C<T>* out, int nf, T scale, int x0 = 0,
                                         int nx = N, int kcut = N / 2, int r2 = -1) {
     const int kid = AXIS == 1 ? LD_K_YPASS : -2;
@@ -451,7 +469,17 @@ struct Engine final : EngineBase {
     const int qoff = AXIS == 1 ? x0 * lw : 0, qend = AXIS == 1 ? (x0 + nx) * lw : 0;
     const int nl = nx * lw;
     LD_TRY(prof_mark(kid, s));
-    if (AXIS == 1 && my == MH)
+    if (AXIS == 1 && ysplit) {
+      if (my == MH)
+        hipLaunchKernelGGL((k_pass<T, N, MH, AXIS, DIR, NQ>), dim3((nl + MH - 1) / MH, nf, B), dim3(MH * NQ), 0, s,
+                           in, out, CF, (const C<T>*)tw, scale, qoff, rs, lw, qend, r2);
+      else if (my == M2)
+        hipLaunchKernelGGL((k_pass<T, N, M2, AXIS, DIR, NQ>), dim3((nl + M2 - 1) / M2, nf, B), dim3(M2 * NQ), 0, s,
+                           in, out, CF, (const C<T>*)tw, scale, qoff, rs, lw, qend, r2);
+      else
+        hipLaunchKernelGGL((k_pass<T, N, M, AXIS, DIR, NQ>), dim3((nl + M - 1) / M, nf, B), dim3(M * NQ), 0, s,
+                           in, out, CF, (const C<T>*)tw, scale, qoff, rs, lw, qend, r2);
+    } else if (AXIS == 1 && my == MH)
       hipLaunchKernelGGL((k_pass<T, N, MH, AXIS, DIR>), dim3((nl + MH - 1) / MH, nf, B), dim3(MH * N1), 0, s, in, out,
                          CF, (const C<T>*)tw, scale, qoff, rs, lw, qend, r2);
     else if (AXIS == 1 && my == M2)

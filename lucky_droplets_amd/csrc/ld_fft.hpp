@@ -17,12 +17,15 @@
 
 namespace ld {
 
-template <int N> struct Plan {
+// Q = 0: the default split N1 = 2^floor(p/2); Q > 0: N1 = Q threads per line
+// (a power of two with Q * Q <= N, so that N2 >= N1).
+template <int N, int Q = 0> struct Plan {
   static constexpr int lg = (N == 16) ? 4 : (N == 32) ? 5 : (N == 64) ? 6 : (N == 128) ? 7
                           : (N == 256) ? 8 : (N == 512) ? 9 : -1;
   static_assert(lg > 0, "N must be a power of two in [16, 512]");
-  static constexpr int N1 = 1 << (lg / 2);
+  static constexpr int N1 = Q ? Q : 1 << (lg / 2);
   static constexpr int N2 = N / N1;
+  static_assert(N1 >= 2 && N2 % N1 == 0, "split N = N1 * N2 with N1 <= N2");
   static constexpr int S = N2 / N1;     // step-2 DFTs per thread
   static constexpr int Nf = N / 2 + 1;
 };
@@ -83,19 +86,19 @@ template <typename T, int DIR> struct Dft<T, 2, DIR> {
 };
 
 // Element index held in register r by thread t after a forward transform.
-template <int N> __device__ __forceinline__ int posOut(int t, int r) {
-  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2;
+template <int N, int Q = 0> __device__ __forceinline__ int posOut(int t, int r) {
+  constexpr int N1 = Plan<N, Q>::N1, N2 = Plan<N, Q>::N2;
   return (t + N1 * (r / N1)) + N2 * (r % N1);
 }
 // Element index held in register n2 by thread n1 before a forward transform.
-template <int N> __device__ __forceinline__ int posIn(int n1, int n2) {
-  return n1 + Plan<N>::N1 * n2;
+template <int N, int Q = 0> __device__ __forceinline__ int posIn(int n1, int n2) {
+  return n1 + Plan<N, Q>::N1 * n2;
 }
 
 // LDS swizzle of the exchange: element (k2, n1) of a line lives at
 // k2*N1 + ((n1 + k2) mod N1).
-template <int N> __device__ __forceinline__ int xchg(int k2, int n1) {
-  constexpr int N1 = Plan<N>::N1;
+template <int N, int Q = 0> __device__ __forceinline__ int xchg(int k2, int n1) {
+  constexpr int N1 = Plan<N, Q>::N1;
   return k2 * N1 + ((n1 + k2) & (N1 - 1));
 }
 
@@ -104,9 +107,9 @@ template <int N> __device__ __forceinline__ int xchg(int k2, int n1) {
 // line's LDS scratch (N elements).  Every thread of the block must call it
 // (it contains block barriers); threads with active == false only sync.
 // tw[m] = exp(-2 pi i m / N).
-template <typename T, int N>
+template <typename T, int N, int Q = 0>
 __device__ __forceinline__ void lineFwd(C<T>* a, C<T>* L, int n1, bool active, const C<T>* __restrict__ tw) {
-  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, S = Plan<N>::S;
+  constexpr int N1 = Plan<N, Q>::N1, N2 = Plan<N, Q>::N2, S = Plan<N, Q>::S;
   if (active) {
     Dft<T, N2, -1>::run(a);
 #pragma unroll
@@ -117,7 +120,7 @@ __device__ __forceinline__ void lineFwd(C<T>* a, C<T>* L, int n1, bool active, c
   __syncthreads();
   if (active) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) L[xchg<N>(k2, n1)] = a[k2];
+    for (int k2 = 0; k2 < N2; ++k2) L[xchg<N, Q>(k2, n1)] = a[k2];
   }
   __syncthreads();
   if (active) {
@@ -125,7 +128,7 @@ __device__ __forceinline__ void lineFwd(C<T>* a, C<T>* L, int n1, bool active, c
     for (int s = 0; s < S; ++s) {
       const int k2 = n1 + N1 * s;
 #pragma unroll
-      for (int j = 0; j < N1; ++j) a[s * N1 + j] = L[xchg<N>(k2, j)];
+      for (int j = 0; j < N1; ++j) a[s * N1 + j] = L[xchg<N, Q>(k2, j)];
       Dft<T, N1, -1>::run(a + s * N1);
     }
   }
@@ -133,9 +136,9 @@ __device__ __forceinline__ void lineFwd(C<T>* a, C<T>* L, int n1, bool active, c
 
 // Inverse DFT (exp(+2 pi i jk/N), unnormalised) of one line: consumes the
 // posOut order, produces the posIn order.  Same barrier rules as lineFwd.
-template <typename T, int N>
+template <typename T, int N, int Q = 0>
 __device__ __forceinline__ void lineInv(C<T>* a, C<T>* L, int n1, bool active, const C<T>* __restrict__ tw) {
-  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, S = Plan<N>::S;
+  constexpr int N1 = Plan<N, Q>::N1, N2 = Plan<N, Q>::N2, S = Plan<N, Q>::S;
   if (active) {
 #pragma unroll
     for (int s = 0; s < S; ++s) Dft<T, N1, +1>::run(a + s * N1);
@@ -146,13 +149,13 @@ __device__ __forceinline__ void lineInv(C<T>* a, C<T>* L, int n1, bool active, c
     for (int s = 0; s < S; ++s) {
       const int k2 = n1 + N1 * s;
 #pragma unroll
-      for (int j = 0; j < N1; ++j) L[xchg<N>(k2, j)] = a[s * N1 + j];
+      for (int j = 0; j < N1; ++j) L[xchg<N, Q>(k2, j)] = a[s * N1 + j];
     }
   }
   __syncthreads();
   if (active) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) a[k2] = L[xchg<N>(k2, n1)];
+    for (int k2 = 0; k2 < N2; ++k2) a[k2] = L[xchg<N, Q>(k2, n1)];
 #pragma unroll
     for (int k2 = 1; k2 < N2; ++k2) {
       if (n1 != 0) a[k2] = cmul<T>(a[k2], cconj<T>(tw[n1 * k2]));

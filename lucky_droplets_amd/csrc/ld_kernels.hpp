@@ -117,11 +117,11 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
 // removes).  With r2 >= 0, only the spectral entries with ky^2 + kz^2 <= r2
 // are moved: a forward pass writes only those (the next pass reads no other),
 // an inverse pass reads only those (the others are zero).
-template <typename T, int N, int M, int AXIS, int DIR>
-__global__ void __launch_bounds__(M * Plan<N>::N1)
+template <typename T, int N, int M, int AXIS, int DIR, int Q = 0>
+__global__ void __launch_bounds__((M * Plan<N, Q>::N1))
 k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T scale, int qoff, size_t rs,
        int lw, int qend, int r2) {
-  constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, js = JS<N, AXIS>::v;
+  constexpr int N1 = Plan<N, Q>::N1, N2 = Plan<N, Q>::N2, js = JS<N, AXIS>::v;
   __shared__ C<T> lds[M * (N + 1)];
   const int m = threadIdx.x % M, n1 = threadIdx.x / M;
   // XCD-aware: consecutive line groups go to one XCD, so row segments that
@@ -150,30 +150,30 @@ k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T
   if constexpr (DIR < 0) {
     if (live) {
 #pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(src, lb + (uint32_t)(posIn<N>(n1, n2) * js));
+      for (int n2 = 0; n2 < N2; ++n2) a[n2] = ld32(src, lb + (uint32_t)(posIn<N, Q>(n1, n2) * js));
     } else {
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) a[n2] = mkc<T>((T)0, (T)0);
     }
-    lineFwd<T, N>(a, L, n1, true, tw);
+    lineFwd<T, N, Q>(a, L, n1, true, tw);
     if (live) {
 #pragma unroll
       for (int r = 0; r < N2; ++r)
-        if (keep(posOut<N>(n1, r))) st32(dst, lb + (uint32_t)(posOut<N>(n1, r) * js), a[r]);
+        if (keep(posOut<N, Q>(n1, r))) st32(dst, lb + (uint32_t)(posOut<N, Q>(n1, r) * js), a[r]);
     }
   } else {
     if (live) {
 #pragma unroll
       for (int r = 0; r < N2; ++r)
-        a[r] = keep(posOut<N>(n1, r)) ? ld32(src, lb + (uint32_t)(posOut<N>(n1, r) * js)) : mkc<T>((T)0, (T)0);
+        a[r] = keep(posOut<N, Q>(n1, r)) ? ld32(src, lb + (uint32_t)(posOut<N, Q>(n1, r) * js)) : mkc<T>((T)0, (T)0);
     } else {
 #pragma unroll
       for (int r = 0; r < N2; ++r) a[r] = mkc<T>((T)0, (T)0);
     }
-    lineInv<T, N>(a, L, n1, true, tw);
+    lineInv<T, N, Q>(a, L, n1, true, tw);
     if (live) {
 #pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) st32(dst, lb + (uint32_t)(posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
+      for (int n2 = 0; n2 < N2; ++n2) st32(dst, lb + (uint32_t)(posIn<N, Q>(n1, n2) * js), cscale<T>(a[n2], scale));
     }
   }
 }

@@ -262,13 +262,14 @@ class ForcedDNS:
 
     def tuning_all(self):
         """Every launch-geometry choice: (dict of choices, dict of timings in us)."""
-        ch = (ctypes.c_int32 * 6)()
-        us = (ctypes.c_double * 18)()
+        ch = (ctypes.c_int32 * 7)()
+        us = (ctypes.c_double * 19)()
         _lib.check(self.lib.ld_tuning_all(self.ctx, ch, us))
-        names = ["y_lines", "z_cross_tile", "z_vel_tile", "x_uw_lines", "x_rhs_lines", "x_kn_lines"]
+        names = ["y_lines", "z_cross_tile", "z_vel_tile", "x_uw_lines", "x_rhs_lines", "x_kn_lines", "y_split_n1"]
         times = {"y_lines": list(us[0:3])}
-        for k, nm in enumerate(names[1:]):
+        for k, nm in enumerate(names[1:6]):
             times[nm] = [us[3 + 3 * k], us[4 + 3 * k], us[5 + 3 * k]]
+        times["y_half_split_at_chosen_lines"] = [us[18]]
         return dict(zip(names, ch[:])), times
 
     def status_all(self):

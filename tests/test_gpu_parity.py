@@ -689,7 +689,42 @@ def test_every_launch_geometry_gives_the_same_bits(gpu, N, prec, monkeypatch):
     assert choice["y_lines"] in (4, 8, 16, 32)
     for k in ("z_cross_tile", "z_vel_tile", "x_uw_lines", "x_rhs_lines", "x_kn_lines"):
         assert choice[k] >= 2 and times[k][0] > 0
+    assert choice["y_split_n1"] in (4, 8, 16) and times["y_half_split_at_chosen_lines"][0] > 0
     d.close()
+
+
+@pytest.mark.parametrize("N,prec", [(64, "f32"), (32, "f64"), (256, "f32")])
+def test_y_pass_other_line_split_matches_default(gpu, N, prec, monkeypatch):
+    """The y passes with the other line split (N1/2 threads per line, each
+    holding 2 N2 elements; LD_YSPLIT=1) give the default split's two steps to
+    rounding (another factorisation: not bit for bit): 1e-13 relative in norm
+    for FP64, 2e-6 for FP32, on u_hat, n and the droplet rows."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(N, prec, 3000 if N < 256 else 20000, nu=0.01)
+    rng = np.random.default_rng(32)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, _ = ic.initial_droplets(cfg, rng)
+    outs = []
+    for split in ("0", "1"):
+        monkeypatch.setenv("LD_YSPLIT", split)
+        d = ForcedDNS(cfg)
+        choice, _ = d.tuning_all()
+        t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+        d.set_state(*t)
+        d.advance(dt, 2)
+        d.get_state(*t)
+        torch.cuda.synchronize()
+        outs.append(([x.cpu().numpy() for x in t], choice["y_split_n1"]))
+        d.close()
+    monkeypatch.delenv("LD_YSPLIT")
+    (a, n1a), (b, n1b) = outs
+    assert n1b * 2 == n1a
+    tol = 1e-13 if prec == "f64" else 2e-6
+    for x, y in zip(a, b):
+        assert relerr(y, x) < tol
 
 
 def test_c1_200_steps_against_oracle_golden(gpu):
