@@ -503,7 +503,10 @@ struct Engine final : EngineBase {
     return LD_OK;
   }
   template <class X> int fill(hipStream_t s, X* x, size_t n, X val) {
-    hipLaunchKernelGGL((k_fill<X>), dim3(1024, 1, B), dim3(256), 0, s, x, n, val, rs);
+    // grid-stride fill; small fills (counters, flags) take as few blocks as they need
+    const size_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL((k_fill<X>), dim3(nb < 1024 ? (unsigned)(nb ? nb : 1) : 1024u, 1, B), dim3(256), 0, s, x, n,
+                       val, rs);
     LD_LAUNCHED();
     return LD_OK;
   }
