@@ -7,7 +7,8 @@ E(k) ~ k^4 exp(-2 (k/3)^2) random-phase initial field (seed 0), droplet St
 lognormal (median 0.3, sigma 0.6), dt from CFL 0.5.  Inputs are generated on
 the host, copied to HBM before timing; a step is one full RK4 step (four RHS
 evaluations, droplet advance, clip_zero, reality round trip, projection,
-blow-up check) advanced on the device-resident state.
+blow-up check) advanced on the device-resident state.  Defaults: 10 warm-up
+steps, then 100 timed steps (HIP events on the stream).
 
 Prints ONE JSON line.
 """
@@ -289,8 +290,8 @@ def cpu_baseline(uk, n, coord, cfg, dt, c1_steps=10):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
