@@ -17,7 +17,7 @@ spectrum truncated to the coarse grid (ld_spectral_resample), the target the
 fine velocity minus base (ld_subgrid_residual), i.e. the modes the coarse grid
 cannot carry.  The droplets of every form are advanced one-way coupled in a
 frozen velocity field per step (ld_droplets_frozen_rk4, the drag and settling
-of particles.py:831-835): the fine DNS field (the judge), the coarse field on
+of particles.py:831-835): the fine DNS field (the reference form), the coarse field on
 the coarse grid (plain interpolation, the fallback), or u_rec (the learned
 form).  Pair statistics (ld_pair_stats) give g(r) and the kinematic collision
 kernel of each form.
