@@ -100,3 +100,43 @@ def test_state_file_umat_is_fluid_velocity_at_droplets(gpu, tmp_path):
     order = np.argsort(st["prtclid"].reshape(-1))
     assert relerr(st["umat"][order], ref[np.argsort(pid)]) < 1e-12
     d.close()
+
+
+def test_kernel_class_timing_counts_launches_and_keeps_results(gpu):
+    """ld_profile (the bench's per-kernel HIP-event timing): every class
+    records one launch per RK stage (the y passes five per stage: u/omega and
+    v inverse, nonlinear and density forward, kn inverse), a positive total
+    duration, and the steps taken while profiling equal unprofiled ones bit for
+    bit."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(32, "f32", 2000)
+    rng = np.random.default_rng(8)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, _ = ic.initial_droplets(cfg, rng)
+    per_stage = {"droplets": 1, "deposit": 1, "x_rhs": 1, "x_uw": 1, "y_pass": 5, "z_cross": 1, "z_vel": 1,
+                 "x_kn": 1, "z_kn": 1}
+    ref = None
+    for name in [None] + list(per_stage):
+        d = ForcedDNS(cfg)
+        t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+        d.set_state(*t)
+        if name:
+            d.profile(name, True)
+        d.advance(dt, 2)
+        if name:
+            ms, launches = d.profile_read()
+            d.profile(name, False)
+            assert launches == 2 * 4 * per_stage[name], (name, launches)
+            assert ms > 0.0
+        d.get_state(*t)
+        torch.cuda.synchronize()
+        out = [x.cpu().numpy() for x in t]
+        d.close()
+        if ref is None:
+            ref = out
+        else:
+            for a, b in zip(ref, out):
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
