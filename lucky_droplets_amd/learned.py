@@ -18,8 +18,9 @@ fine velocity minus base (ld_subgrid_residual), i.e. the modes the coarse grid
 cannot carry.  The droplets of every form are advanced one-way coupled in a
 frozen velocity field per step (ld_droplets_frozen_rk4, the drag and settling
 of particles.py:831-835): the fine DNS field (the reference form), the coarse field on
-the coarse grid (plain interpolation, the fallback), or u_rec (the learned
-form).  Pair statistics (ld_pair_stats) give g(r) and the kinematic collision
+the coarse grid (plain interpolation), or u_rec (the learned form, which
+falls back to the base alone when the trained network does worse than none
+on a held-out state).  Pair statistics (ld_pair_stats) give g(r) and the kinematic collision
 kernel of each form.
 """
 from __future__ import annotations
@@ -57,6 +58,7 @@ class SubgridModel:
         # last layer zero: training starts from the plain reconstruction (base)
         self.net.init_params(seed=seed, gain=0.0)
         self.sx = self.sg = self.sy = 1.0
+        self.fallback = False       # set by validate(): the network is not used
         self._ukf = self.fine.zeros_spec(3)
         self._base = self.fine.zeros_real(3)
         self._f12 = self.coarse.zeros_real(12)
@@ -104,10 +106,11 @@ class SubgridModel:
         return x, t
 
     def reconstruct(self, uk_c: torch.Tensor, out: Optional[torch.Tensor] = None, learned: bool = True):
-        """u_rec = base + sy net(x) on the fine grid (learned=False: base only)."""
+        """u_rec = base + sy net(x) on the fine grid (learned=False, or a model
+        whose network did worse than none on held-out data: base only)."""
         out = self.fine.zeros_real(3) if out is None else out
         base = self.base(uk_c)
-        if not learned:
+        if not learned or self.fallback:
             out.copy_(base)
             return out
         y = self.net.forward(self.inputs(uk_c))
@@ -142,6 +145,13 @@ class SubgridModel:
             a = lr * (lr_end / lr) ** (it / max(1, iters - 1))
             losses.append(self.net.train_step(x, t, lr=a))
         return losses
+
+    def validate(self, uk_fine: torch.Tensor):
+        """Held-out errors (plain, learned) of one fine state; the model falls
+        back to the plain reconstruction when the network's is not smaller."""
+        e_plain, e_learned = self.heldout_error(uk_fine)
+        self.fallback = not e_learned < e_plain
+        return e_plain, e_learned
 
     def heldout_error(self, uk_fine: torch.Tensor):
         """Mean squared subgrid error at the fine nodes, (plain, learned): plain
@@ -228,7 +238,7 @@ def collision_comparison(cfg_fine: Config, npart: int, st: float, spinup: int, t
     pairs = [model.pair(s) for s in states]
     pairs = [(x.clone(), t.clone()) for x, t in pairs]
     losses = model.train(pairs, train_iters)
-    e_plain, e_learned = model.heldout_error(heldout)
+    e_plain, e_learned = model.validate(heldout)
     log(f"[learned] trained on {len(states)} states, {train_iters} steps in {time.time() - t0:.1f}s; "
         f"held-out subgrid mse plain {e_plain:.3e} learned {e_learned:.3e}")
     del pairs, states
@@ -278,6 +288,7 @@ def collision_comparison(cfg_fine: Config, npart: int, st: float, spinup: int, t
             for k, rws in forms.items():
                 stats[k].append(pair_stats(fine, rws, cfg_fine.L, rmax, nbins))
     out = {"dt": dt, "steps": nsteps, "heldout_mse_plain": e_plain, "heldout_mse_learned": e_learned,
+           "network_used": not model.fallback,
            "train_loss_first": float(np.mean(losses[:10])), "train_loss_last": float(np.mean(losses[-10:])),
            "ms_per_step": {k: 1e3 * v / nsteps for k, v in times.items()}}
     res = {}

@@ -122,6 +122,26 @@ def test_frozen_field_droplets_match_rk4_with_oracle_gather(gpu, prec):
     d.close()
 
 
+def test_untrained_model_falls_back_to_the_base(gpu):
+    """A network that does no better than none on held-out data is not used:
+    the untrained model (last layer zero) validates as a fallback and its
+    reconstruction is the base alone, bit for bit."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.learned import SubgridModel
+    cfg = Config(N=32, precision="f32", nu=0.02, nshells=2, profile="k4")
+    m = SubgridModel(cfg)
+    uk, _, _ = ic.initial_flow(cfg, np.random.default_rng(9))
+    ukf = torch.from_numpy(uk).to(gpu).to(torch.complex64)
+    e_plain, e_learned = m.validate(ukf)
+    assert e_learned == e_plain and m.fallback
+    ukc = m.coarse_spectrum(ukf)
+    rec = m.reconstruct(ukc).clone()
+    base = m.base(ukc).clone()
+    assert torch.equal(rec, base)
+    m.close()
+
+
 def test_learned_form_collision_kernel_against_fine_dns(gpu):
     """128^3 fine DNS against the 64^3 coarse DNS with plain interpolation and
     with the learned reconstruction, 2 x 10^5 droplets (St 0.5): the learned
@@ -139,4 +159,4 @@ def test_learned_form_collision_kernel_against_fine_dns(gpu):
         assert np.all(np.isfinite(f["K"])) and np.all(f["counts"] > 0)
     assert r["kernel_err_learned"] < 0.05
     assert r["kernel_err_learned"] < r["kernel_err_plain"]
-    assert r["heldout_mse_learned"] < 0.95 * r["heldout_mse_plain"]
+    assert r["heldout_mse_learned"] < 0.95 * r["heldout_mse_plain"] and r["network_used"]
