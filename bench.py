@@ -97,12 +97,13 @@ def time_gpu(cfg, uk, n, coord, dt, steps, warmup, profile_kernel=None, graph=Fa
     return ms, kern, lastmax, blown
 
 
-def time_large(cfg, uk_small, n_small, dt_small, steps, warmup):
-    """ms per step of the 512^3 FP32 + 10^7-droplet workload.  Its state is the
-    smaller workload's initial field zero-padded to this grid on the device
-    (ld_spectral_resample: the same band-limited velocity field, so the
+def large_context(cfg, uk_small, dt_small):
+    """The 512^3 FP32 + 10^7-droplet workload as a resident context.  Its state
+    is the smaller workload's initial field zero-padded to this grid on the
+    device (ld_spectral_resample: the same band-limited velocity field, so the
     dealiasing fast path applies) with a uniform density and the workload's own
-    seeded droplet population; dt scales with the grid spacing."""
+    seeded droplet population; dt scales with the grid spacing.  Returns
+    (context, dt)."""
     import numpy as np
     import torch
     from lucky_droplets_amd import _lib, ic
@@ -121,7 +122,13 @@ def time_large(cfg, uk_small, n_small, dt_small, steps, warmup):
     del coord
     d.set_state(uk, n, cg)
     del uk, n, cg
-    dt = dt_small * ns / cfg.N
+    return d, dt_small * ns / cfg.N
+
+
+def time_large(cfg, uk_small, dt_small, steps, warmup):
+    """ms per step of the 512^3 FP32 + 10^7-droplet workload (large_context)."""
+    import torch
+    d, dt = large_context(cfg, uk_small, dt_small)
     d.advance(dt, warmup)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -337,6 +344,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64 x 64^3 ensemble timing")
     ap.add_argument("--no-large", action="store_true", help="skip the 512^3 + 10^7-droplet timing")
+    ap.add_argument("--track-steps", type=int, default=0,
+                    help="also run the 512^3 workload this many steps with mass tracking (off by default)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-against-FP64 accuracy run")
     ap.add_argument("--no-collisions", action="store_true", help="skip the collision-pair timing")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-subgrid-form comparison")
@@ -474,7 +483,7 @@ def main():
         from lucky_droplets_amd.config import preset as _preset
         c4 = _preset("c4")
         log(f"[bench] large: {c4.N}^3 FP32 + {c4.n_particles} droplets")
-        ms4, dt4, blown4 = time_large(c4, uk, n, dt, max(5, args.steps // 10), 2)
+        ms4, dt4, blown4 = time_large(c4, uk, dt, max(5, args.steps // 10), 2)
         b4 = algorithmic_bytes(c4.N, 4, c4.n_particles, 40)
         out["large"] = {"workload": f"forced DNS {c4.N}^3 FP32 + {c4.n_particles} droplets (bidisperse St 0.2 / 0.6, "
                                     f"nu {c4.nu}); the {cfg.N}^3 initial field zero-padded, uniform density",
@@ -483,6 +492,16 @@ def main():
                         "blowup": blown4}
         blown = blown or blown4
         log(f"[bench] large {ms4:.2f} ms/step")
+        if args.track_steps > 0:
+            from lucky_droplets_amd import tracking
+            d4, dt4 = large_context(c4, uk, dt)
+            hist = tracking.mass_history(d4, dt4, args.track_steps, max(1, args.track_steps // 20), log=log)
+            d4.close()
+            out["large"]["mass_tracking"] = tracking.summary(hist)
+            out["large"]["mass_tracking"]["history"] = hist.tolist()
+            log(f"[bench] large: {args.track_steps} steps tracked, droplet mass "
+                f"{hist[0, 1]:.6e} -> {hist[-1, 1]:.6e}, max drift of the conserved sum "
+                f"{out['large']['mass_tracking']['max_total_drift']:.3e}")
     if not args.no_ensemble:
         from lucky_droplets_amd.config import preset as _preset
         c3 = _preset("c3")
