@@ -140,3 +140,26 @@ def test_kernel_class_timing_counts_launches_and_keeps_results(gpu):
         else:
             for a, b in zip(ref, out):
                 assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_mass_history_conserves_the_sum(gpu, prec):
+    """tracking.mass_history over 20 steps with growing droplets: the
+    droplets gain mass, the density loses it, and the sum stays within 1e-12
+    (FP64) / 1e-5 (FP32) of the mass exchanged."""
+    import torch
+    from lucky_droplets_amd import ic, tracking
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(32, prec, 4000)
+    rng = np.random.default_rng(12)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, _ = ic.initial_droplets(cfg, rng)
+    d = ForcedDNS(cfg)
+    d.set_state(_t(uk, gpu, d.cdtype), _t(np.ones_like(n), gpu, d.rdtype), _t(coord, gpu, torch.float64))
+    hist = tracking.mass_history(d, dt, 20, 5)
+    d.close()
+    assert hist.shape == (5, 4) and list(hist[:, 0]) == [0, 5, 10, 15, 20]
+    s = tracking.summary(hist)
+    assert s["droplet_mass_gained"] > 0 and hist[-1, 2] < hist[0, 2]
+    assert s["max_total_drift_per_gained"] < (1e-12 if prec == "f64" else 1e-5)
