@@ -891,15 +891,31 @@ k_bin_count(const double* __restrict__ pos0, int np, int cap, double L, double i
   int* __restrict__ rank = rsel1(rank0, rs);
   int* __restrict__ hist = rsel1(hist0, rs);
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  int b[3];
+  const int lane = threadIdx.x & 63;
+  const bool valid = p < np;
+  int k = -1 - lane;                        // invalid lanes: keys no other lane has
+  if (valid) {
+    int b[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    b[c] = cellOf<N>(wrapFast(pos[c * cap + p], L, invL), inv_dx) / Bins<N>::BS;
+    for (int c = 0; c < 3; ++c) b[c] = cellOf<N>(wrapFast(pos[c * cap + p], L, invL), inv_dx) / Bins<N>::BS;
+    k = Bins<N>::key(b[0], b[1], b[2]);
+    key[p] = k;
   }
-  const int k = Bins<N>::key(b[0], b[1], b[2]);
-  key[p] = k;
-  rank[p] = atomicAdd(&hist[k], 1);
+  // The slots are in last step's bin order, so lanes with one key mostly form
+  // runs: the first lane of each run of equal keys adds the run's length to
+  // the bin count and the run takes consecutive ranks from the old count
+  // (one atomic per run instead of one per droplet).
+  const int kprev = __shfl_up(k, 1, 64);     // every lane takes part in the shuffle
+  const bool head = lane == 0 || kprev != k;
+  const unsigned long long hm = __ballot(head);
+  const unsigned long long upto = (2ull << lane) - 1ull;         // bits 0 .. lane
+  const int start = 63 - __clzll(hm & upto);
+  const unsigned long long after = hm & ~upto;
+  const int next = after ? __ffsll((long long)after) - 1 : 64;
+  int base = 0;
+  if (head && valid) base = atomicAdd(&hist[k], next - lane);
+  base = __shfl(base, start, 64);
+  if (valid) rank[p] = base + (lane - start);
 }
 
 // Exclusive scan of 4096 ints per block (1024 threads x 4); block totals to bsum.
