@@ -378,18 +378,14 @@ struct OpZFwd {
   __device__ C<T>* specOut(int f) const { return out + f * fout; }
   __device__ void physical(const ZTile<T, N, P>& t, int line0) const {
     using Q = Quads<ZThreads<N, P, 0, F>::v, F * P * N / 4>;
-    V4<T> q[Q::IT];
+    // one loop (global loads straight into the tile): with a separate register
+    // array the compiler kept it in scratch memory
 #pragma unroll
     for (int it = 0; it < Q::IT; ++it) {
-      if (!Q::ok(it)) break;
-      const int e4 = 4 * Q::e(it), f = e4 / (P * N), rem = e4 % (P * N), p = rem / N, j = rem % N;
-      q[it] = *reinterpret_cast<const V4<T>*>(in + f * fin + (size_t)(line0 + p) * N + j);
-    }
-#pragma unroll
-    for (int it = 0; it < Q::IT; ++it) {
-      if (!Q::ok(it)) break;
-      const int e4 = 4 * Q::e(it), f = e4 / (P * N), rem = e4 % (P * N), p = rem / N, j = rem % N;
-      t.re4(f, p, j) = q[it];
+      if (Q::ok(it)) {
+        const int e4 = 4 * Q::e(it), f = e4 / (P * N), rem = e4 % (P * N), p = rem / N, j = rem % N;
+        t.re4(f, p, j) = *reinterpret_cast<const V4<T>*>(in + f * fin + (size_t)(line0 + p) * N + j);
+      }
     }
   }
 };
