@@ -106,15 +106,17 @@ template <int N, int Q = 0> __device__ __forceinline__ int xchg(int k2, int n1) 
 // x[posIn(n1, n2)]; on exit register r holds X[posOut(n1, r)].  L is this
 // line's LDS scratch (N elements).  Every thread of the block must call it
 // (it contains block barriers); threads with active == false only sync.
-// tw[m] = exp(-2 pi i m / N).
-template <typename T, int N, int Q = 0>
-__device__ __forceinline__ void lineFwd(C<T>* a, C<T>* L, int n1, bool active, const C<T>* __restrict__ tw) {
+// The inter-step twiddle W_N^(n1 k2) comes from twd(k2): a load of the table
+// tw[m] = exp(-2 pi i m / N) (lineFwd), or registers the caller filled
+// earlier (lineFwdW).
+template <typename T, int N, int Q, class TW>
+__device__ __forceinline__ void lineFwdT(C<T>* a, C<T>* L, int n1, bool active, TW&& twd) {
   constexpr int N1 = Plan<N, Q>::N1, N2 = Plan<N, Q>::N2, S = Plan<N, Q>::S;
   if (active) {
     Dft<T, N2, -1>::run(a);
 #pragma unroll
     for (int k2 = 1; k2 < N2; ++k2) {
-      if (n1 != 0) a[k2] = cmul<T>(a[k2], tw[n1 * k2]);
+      if (n1 != 0) a[k2] = cmul<T>(a[k2], twd(k2));
     }
   }
   __syncthreads();
@@ -133,11 +135,27 @@ __device__ __forceinline__ void lineFwd(C<T>* a, C<T>* L, int n1, bool active, c
     }
   }
 }
+template <typename T, int N, int Q = 0>
+__device__ __forceinline__ void lineFwd(C<T>* a, C<T>* L, int n1, bool active, const C<T>* __restrict__ tw) {
+  lineFwdT<T, N, Q>(a, L, n1, active, [&](int k2) { return tw[n1 * k2]; });
+}
+// w[k2] = tw[n1 k2] already in registers (k2 = 1 .. N2 - 1)
+template <typename T, int N, int Q = 0>
+__device__ __forceinline__ void lineFwdW(C<T>* a, C<T>* L, int n1, bool active, const C<T>* w) {
+  lineFwdT<T, N, Q>(a, L, n1, active, [&](int k2) { return w[k2]; });
+}
+// Twiddles of thread n1 for both directions, loaded into registers.
+template <typename T, int N, int Q = 0>
+__device__ __forceinline__ void loadTwiddles(C<T>* w, int n1, const C<T>* __restrict__ tw) {
+#pragma unroll
+  for (int k2 = 1; k2 < Plan<N, Q>::N2; ++k2) w[k2] = tw[n1 * k2];
+}
 
 // Inverse DFT (exp(+2 pi i jk/N), unnormalised) of one line: consumes the
-// posOut order, produces the posIn order.  Same barrier rules as lineFwd.
-template <typename T, int N, int Q = 0>
-__device__ __forceinline__ void lineInv(C<T>* a, C<T>* L, int n1, bool active, const C<T>* __restrict__ tw) {
+// posOut order, produces the posIn order.  Same barrier rules as lineFwd;
+// twd(k2) as there (the conjugate is taken here).
+template <typename T, int N, int Q, class TW>
+__device__ __forceinline__ void lineInvT(C<T>* a, C<T>* L, int n1, bool active, TW&& twd) {
   constexpr int N1 = Plan<N, Q>::N1, N2 = Plan<N, Q>::N2, S = Plan<N, Q>::S;
   if (active) {
 #pragma unroll
@@ -158,10 +176,46 @@ __device__ __forceinline__ void lineInv(C<T>* a, C<T>* L, int n1, bool active, c
     for (int k2 = 0; k2 < N2; ++k2) a[k2] = L[xchg<N, Q>(k2, n1)];
 #pragma unroll
     for (int k2 = 1; k2 < N2; ++k2) {
-      if (n1 != 0) a[k2] = cmul<T>(a[k2], cconj<T>(tw[n1 * k2]));
+      if (n1 != 0) a[k2] = cmul<T>(a[k2], cconj<T>(twd(k2)));
     }
     Dft<T, N2, +1>::run(a);
   }
 }
+// PRE: the twiddles are loaded on entry, so their latency overlaps the first
+// step and the exchange instead of following the second barrier (2 N2 more
+// registers live across the exchange).
+template <typename T, int N, int Q = 0, bool PRE = false>
+__device__ __forceinline__ void lineInv(C<T>* a, C<T>* L, int n1, bool active, const C<T>* __restrict__ tw) {
+  if constexpr (PRE) {
+    C<T> w[Plan<N, Q>::N2];
+    loadTwiddles<T, N, Q>(w, active ? n1 : 0, tw);
+    lineInvT<T, N, Q>(a, L, n1, active, [&](int k2) { return w[k2]; });
+  } else {
+    lineInvT<T, N, Q>(a, L, n1, active, [&](int k2) { return tw[n1 * k2]; });
+  }
+}
+template <typename T, int N, int Q = 0>
+__device__ __forceinline__ void lineInvW(C<T>* a, C<T>* L, int n1, bool active, const C<T>* w) {
+  lineInvT<T, N, Q>(a, L, n1, active, [&](int k2) { return w[k2]; });
+}
+
+// Thread n1's twiddles for a kernel that runs several line transforms: held
+// in registers, loaded once at kernel entry so their latency hides behind the
+// first global loads, when the set fits in 32 VGPRs (N2 complex values of at
+// most 128 bytes); above that the registers would cost occupancy or spill,
+// and each step reads the table instead.  Pass to lineFwdT / lineInvT.
+template <typename T, int N, bool REG = (Plan<N>::N2 * sizeof(C<T>) <= 128)>
+struct Twiddles;
+template <typename T, int N> struct Twiddles<T, N, true> {
+  C<T> w[Plan<N>::N2];
+  __device__ __forceinline__ Twiddles(int n1, const C<T>* __restrict__ tw) { loadTwiddles<T, N>(w, n1, tw); }
+  __device__ __forceinline__ C<T> operator()(int k2) const { return w[k2]; }
+};
+template <typename T, int N> struct Twiddles<T, N, false> {
+  const C<T>* __restrict__ tw;
+  int n1;
+  __device__ __forceinline__ Twiddles(int n1_, const C<T>* __restrict__ tw_) : tw(tw_), n1(n1_) {}
+  __device__ __forceinline__ C<T> operator()(int k2) const { return tw[n1 * k2]; }
+};
 
 }  // namespace ld

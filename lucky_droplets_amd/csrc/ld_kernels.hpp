@@ -170,7 +170,7 @@ k_pass(const C<T>* in, C<T>* out, size_t fstride, const C<T>* __restrict__ tw, T
 #pragma unroll
       for (int r = 0; r < N2; ++r) a[r] = mkc<T>((T)0, (T)0);
     }
-    lineInv<T, N, Q>(a, L, n1, true, tw);
+    lineInv<T, N, Q, true>(a, L, n1, true, tw);
     if (live) {
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) st32(dst, lb + (uint32_t)(posIn<N, Q>(n1, n2) * js), cscale<T>(a[n2], scale));
@@ -206,8 +206,9 @@ struct ZTile {
 // Inverse z transform (c2r, unnormalised) of FIN fields already loaded as
 // spectral lines into the tile; leaves real lines in the tile.  Imaginary
 // parts of the kz = 0 and kz = N/2 modes are ignored, as a c2r transform does.
+// w: this thread's twiddles (n1 = threadIdx.x % N1).
 template <typename T, int N, int P, int FIN>
-__device__ __forceinline__ void zInverse(const ZTile<T, N, P>& tile, const C<T>* __restrict__ tw) {
+__device__ __forceinline__ void zInverse(const ZTile<T, N, P>& tile, const Twiddles<T, N>& w) {
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1;
   const int r = threadIdx.x / N1, n1 = threadIdx.x % N1;
   const bool active = r < FIN * P / 2;
@@ -229,7 +230,7 @@ __device__ __forceinline__ void zInverse(const ZTile<T, N, P>& tile, const C<T>*
       a[rr] = mkc<T>(A.x - B.y, A.y + B.x);   // A + i B
     }
   }
-  lineInv<T, N>(a, reg, n1, active, tw);
+  lineInvT<T, N, 0>(a, reg, n1, active, w);
   __syncthreads();
   if (active) {
     T* Rr = (T*)reg;
@@ -245,9 +246,9 @@ __device__ __forceinline__ void zInverse(const ZTile<T, N, P>& tile, const C<T>*
 
 // Forward z transform (r2c, unnormalised) of FOUT real fields in the tile;
 // leaves each pair's complex spectrum Z (natural order, N entries) in its
-// region.  Split with zSplit().
+// region.  Split with zSplit().  w as for zInverse.
 template <typename T, int N, int P, int FOUT>
-__device__ __forceinline__ void zForward(const ZTile<T, N, P>& tile, const C<T>* __restrict__ tw) {
+__device__ __forceinline__ void zForward(const ZTile<T, N, P>& tile, const Twiddles<T, N>& w) {
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2;
   const int r = threadIdx.x / N1, n1 = threadIdx.x % N1;
   const bool active = r < FOUT * P / 2;
@@ -261,7 +262,7 @@ __device__ __forceinline__ void zForward(const ZTile<T, N, P>& tile, const C<T>*
       a[n2] = mkc<T>(Rr[j], Rr[N + j]);
     }
   }
-  lineFwd<T, N>(a, reg, n1, active, tw);
+  lineFwdT<T, N, 0>(a, reg, n1, active, w);
   __syncthreads();
   if (active) {
 #pragma unroll
@@ -312,18 +313,21 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcout) {
   const int line0 = lineoff + blockIdx.x * P;
   const size_t g0 = (size_t)line0 * Nf;
   const int nt = blockDim.x;
+  // the thread's line-FFT twiddles (the same n1 in both directions), loaded
+  // first so that their latency overlaps the tile loads
+  const Twiddles<T, N> w((int)threadIdx.x % Plan<N>::N1, tw);
   if constexpr (FIN > 0) {
     for (int e = threadIdx.x; e < FIN * (NE / 2); e += nt) {
       const int f = e / (NE / 2), v = e % (NE / 2);
       lds2[f * (NE / 2) + v] = reinterpret_cast<const C2<T>*>(op.specIn(f) + g0)[v];
     }
     __syncthreads();
-    zInverse<T, N, P, FIN>(tile, tw);
+    zInverse<T, N, P, FIN>(tile, w);
   }
   op.physical(tile, line0);
   __syncthreads();
   if constexpr (FOUT > 0) {
-    zForward<T, N, P, FOUT>(tile, tw);
+    zForward<T, N, P, FOUT>(tile, w);
     for (int e = threadIdx.x; e < FOUT * (NE / 2); e += nt) {
       const int f = e / (NE / 2), q = 2 * (e % (NE / 2));
       const int za = q % Nf, zb = (q + 1) % Nf;
@@ -632,6 +636,7 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
   __shared__ C<T> lds[3 * M * (N + 1)];
   const int f = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
+  const Twiddles<T, N> wt(n1, tw);         // line-FFT twiddles of n1, both directions
   const int q = xPairBlock() * M + m;
   const T ky = (T)kwave(q / Nf, N), kz = (T)(q % Nf);
   C<T>* L[3] = {lds + m * (N + 1), lds + (M + m) * (N + 1), lds + (2 * M + m) * (N + 1)};
@@ -670,13 +675,13 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
     }
   }
   // (lineInv synchronises before it reuses the regions)
-  lineInv<T, N>(w, Lf, n1, true, tw);
+  lineInvT<T, N, 0>(w, Lf, n1, true, wt);
   C<T>* Ow = out + (3 + f) * fs;
   if (wr) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) st32(Ow, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(w[n2], scale));
   }
-  lineInv<T, N>(u, Lf, n1, true, tw);
+  lineInvT<T, N, 0>(u, Lf, n1, true, wt);
   C<T>* Ou = out + f * fs;
   if (wr) {
 #pragma unroll
@@ -719,6 +724,7 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
   __shared__ C<T> lds[3 * M * (N + 1)];
   const int f = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
+  const Twiddles<T, N> wt(n1, tw);         // line-FFT twiddles of n1, both directions
   const int q = xPairBlock() * M + m;
   const int iy = q / Nf, iz = q % Nf;
   C<T>* L0 = lds + m * (N + 1);
@@ -737,7 +743,7 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) a[n2] = mkc<T>((T)0, (T)0);
   }
-  lineFwd<T, N>(a, Lf, n1, true, tw);
+  lineFwdT<T, N, 0>(a, Lf, n1, true, wt);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < N2; ++r) Lf[posOut<N>(n1, r)] = a[r];
@@ -797,7 +803,7 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < N2; ++r) a[r] = Lf[posOut<N>(n1, r)];
-  lineInv<T, N>(a, Lf, n1, true, tw);
+  lineInvT<T, N, 0>(a, Lf, n1, true, wt);
   // v_hat x-inverse: lines kz <= kcut outside the mask radius are zero and the
   // next y pass reads none of their entries; the kz > kcut lines (zero too)
   // are read as they are by the z pass
@@ -906,6 +912,7 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs, in
   __shared__ C<T> lds[4 * M * (N + 1)];
   const int g = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
+  const Twiddles<T, N> wt(n1, tw);         // line-FFT twiddles of n1, both directions
   const int q = xPairBlock() * M + m;
   const int iy = q / Nf, iz = q % Nf;
   C<T>* L0 = lds + m * (N + 1);
@@ -927,7 +934,7 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs, in
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) a[n2] = mkc<T>((T)0, (T)0);
   }
-  lineFwd<T, N>(a, Lg, n1, true, tw);
+  lineFwdT<T, N, 0>(a, Lg, n1, true, wt);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < N2; ++r) Lg[posOut<N>(n1, r)] = a[r];
@@ -946,7 +953,7 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs, in
 #pragma unroll
     for (int r = 0; r < N2; ++r) a[r] = L0[posOut<N>(n1, r)];
   }
-  lineInv<T, N>(a, L0, n1, g == 0, tw);
+  lineInvT<T, N, 0>(a, L0, n1, g == 0, wt);
   if (g == 0 && (live || iz > kcut)) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) st32(w, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
