@@ -247,7 +247,7 @@ struct Engine final : EngineBase {
         vsel[k] = v;
         auto launch = [&]() -> int {
           switch (k) {
-            case 0: return zcross(st, nullptr, grec[0], 0, N);
+            case 0: return zcross(st, nullptr, grec[0], 0, N, kc);
             case 1: return zvel(st, nullptr, grec[0], n_state, clip_none, 0, N);
             case 2: return nonlinear_x(st, uk, kc);
             case 3: return xrhs(st, ra);
@@ -492,12 +492,14 @@ struct Engine final : EngineBase {
     LD_LAUNCHED();
     return LD_OK;
   }
-  // z pass over the (x, y) lines of x-planes [x0, x0 + nx).
+  // z pass over the (x, y) lines of x-planes [x0, x0 + nx); input entries
+  // with kz > kcin are zero (not loaded), output entries with kz > kcout are
+  // not needed.
   template <int P, int FIN, int FOUT, class Op> int zrun(hipStream_t s, const Op& op, int kid = -2, int x0 = 0,
-                                                         int nx = N, int kcout = N / 2) {
+                                                         int nx = N, int kcout = N / 2, int kcin = N / 2) {
     LD_TRY(prof_mark(kid, s));
     hipLaunchKernelGGL((k_z<T, N, P, FIN, FOUT, Op>), dim3(nx * N / P, 1, B), dim3(ZThreads<N, P, FIN, FOUT>::v), 0,
-                       s, op, (const C<T>*)tw, x0 * N, rs, kcout);
+                       s, op, (const C<T>*)tw, x0 * N, rs, kcin, kcout);
     LD_TRY(prof_mark(kid, s));
     LD_LAUNCHED();
     return LD_OK;
@@ -632,24 +634,25 @@ struct Engine final : EngineBase {
     LD_TRY(prof_mark(LD_K_XUW, s));
     return LD_OK;
   }
-  // kcu < N/2: the stage u_hat has no nonzero mode outside the mask
-  int zcross(hipStream_t s, T* uphys, V4<T>* g, int x0, int nx) {
+  // kcu < N/2: the stage u_hat has no nonzero mode outside the mask, so the
+  // y-inverse output (the input here) is zero at kz > kcu
+  int zcross(hipStream_t s, T* uphys, V4<T>* g, int x0, int nx, int kcu) {
     if (vsel[0] == 2) {
       OpZCross<T, N, P6D> oc{w6, CF, uphys, g, inv()};
-      return zrun<P6D, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc);
+      return zrun<P6D, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc, kcu);
     }
     if (vsel[0]) {
       OpZCross<T, N, P6H> oc{w6, CF, uphys, g, inv()};
-      return zrun<P6H, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc);
+      return zrun<P6H, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc, kcu);
     }
     OpZCross<T, N, P6> oc{w6, CF, uphys, g, inv()};
-    return zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc);
+    return zrun<P6, 6, 3>(s, oc, LD_K_ZCROSS, x0, nx, kc, kcu);
   }
   int nonlinear_yz(hipStream_t s, T* uphys, V4<T>* g, int kcu = N / 2) {
     const int nx = N / nslab;
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, w6, w6, 6, inv(), x0, nx, kcu, kcu < N / 2 ? sphere() : -1)));
-      LD_TRY(zcross(s, uphys, g, x0, nx));
+      LD_TRY(zcross(s, uphys, g, x0, nx, kcu));
       LD_TRY((pass<1, -1>(s, w6, w6, 3, (T)1, x0, nx, kc, sphere())));
     }
     return LD_OK;
@@ -661,14 +664,14 @@ struct Engine final : EngineBase {
   int zvel(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip, int x0, int nx) {
     if (vsel[1] == 2) {
       OpZVel<T, N, P3D> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
-      return zrun<P3D, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc);
+      return zrun<P3D, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc, kc);
     }
     if (vsel[1]) {
       OpZVel<T, N, P3H> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
-      return zrun<P3H, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc);
+      return zrun<P3H, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc, kc);
     }
     OpZVel<T, N, P3> ov{w6, wn, CF, vphys, g, ns, clip, inv(), (T)(p.tps * p.g)};
-    return zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc);
+    return zrun<P3, 3, 3>(s, ov, LD_K_ZVEL, x0, nx, kc, kc);
   }
   int velocity(hipStream_t s, T* vphys, V4<T>* g, const T* ns, const double* clip, hipEvent_t wait) {
     const int nx = N / nslab;
@@ -714,7 +717,7 @@ struct Engine final : EngineBase {
     LD_TRY(prof_mark(LD_K_XKN, s));
     for (int x0 = 0; x0 < N; x0 += nx) {
       LD_TRY((pass<1, +1>(s, wn, wn, 1, inv(), x0, nx, kc, sphere())));
-      if (kz) LD_TRY((zrun<P1, 1, 0>(s, *kz, LD_K_ZKN, x0, nx)));
+      if (kz) LD_TRY((zrun<P1, 1, 0>(s, *kz, LD_K_ZKN, x0, nx, N / 2, kc)));
     }
     return LD_OK;
   }

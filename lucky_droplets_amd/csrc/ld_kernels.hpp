@@ -297,11 +297,13 @@ template <typename T> struct alignas(2 * sizeof(C<T>)) C2 { C<T> a, b; };
 // P*Nf entries per field, kept in LDS in the same order (the regions of
 // consecutive line pairs are adjacent), so they move as pairs of entries;
 // line0 * Nf is even (P and the line offset are even), so every pair is
-// aligned.  Output pairs whose two entries both have kz > kcout are not
+// aligned.  Input entries with kz > kcin are zero by the caller's contract
+// (dealiasing truncation: no earlier pass writes them): they are set in LDS,
+// not loaded.  Output pairs whose two entries both have kz > kcout are not
 // written (no later pass reads them).
 template <typename T, int N, int P, int FIN, int FOUT, class Op>
 __global__ void __launch_bounds__((ZThreads<N, P, FIN, FOUT>::v))
-k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcout) {
+k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcin, int kcout) {
   static_assert(P % 2 == 0, "pairs of lines");
   const Op op = op0.shifted(rs);
   constexpr int Nf = N / 2 + 1;
@@ -317,9 +319,30 @@ k_z(Op op0, const C<T>* __restrict__ tw, int lineoff, size_t rs, int kcout) {
   // first so that their latency overlaps the tile loads
   const Twiddles<T, N> w((int)threadIdx.x % Plan<N>::N1, tw);
   if constexpr (FIN > 0) {
-    for (int e = threadIdx.x; e < FIN * (NE / 2); e += nt) {
+    // pair e = f (NE / 2) + v of the block's input goes to lds2[e].  Every load
+    // of a thread is issued before the first LDS store: the loads are
+    // unconditional (a branch around one would wait for it), a pair that is
+    // not needed (both kz > kcin, or past the end) reads the block's first
+    // pair of the field instead, which other lanes load anyway.
+    constexpr int NT = ZThreads<N, P, FIN, FOUT>::v, NPR = FIN * (NE / 2), IT = (NPR + NT - 1) / NT;
+    C2<T> x[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e0 = (int)threadIdx.x + it * NT;
+      const int e = (NPR % NT == 0 || e0 < NPR) ? e0 : 0;
       const int f = e / (NE / 2), v = e % (NE / 2);
-      lds2[f * (NE / 2) + v] = reinterpret_cast<const C2<T>*>(op.specIn(f) + g0)[v];
+      const bool need = (2 * v) % Nf <= kcin || (2 * v + 1) % Nf <= kcin;
+      x[it] = reinterpret_cast<const C2<T>*>(op.specIn(f) + g0)[need ? v : 0];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = (int)threadIdx.x + it * NT;
+      if (NPR % NT == 0 || e < NPR) {
+        const int v = e % (NE / 2);
+        if ((2 * v) % Nf > kcin) x[it].a = mkc<T>((T)0, (T)0);
+        if ((2 * v + 1) % Nf > kcin) x[it].b = mkc<T>((T)0, (T)0);
+        lds2[e] = x[it];
+      }
     }
     __syncthreads();
     zInverse<T, N, P, FIN>(tile, w);
@@ -642,11 +665,12 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
   C<T>* L[3] = {lds + m * (N + 1), lds + (M + m) * (N + 1), lds + (2 * M + m) * (N + 1)};
   C<T>* Lf = lds + (f * M + m) * (N + 1);
   // kcut < N/2: u_hat has no mode outside the mask, so a line with
-  // ky^2 + kz^2 > kcut (kcut + 1) is zero (not read); its outputs are written
-  // only where a later pass reads them as they are (kz > kcut: the z pass)
+  // ky^2 + kz^2 > kcut (kcut + 1) is zero: it is not read, and its zero
+  // outputs are not written (the y pass skips those entries, the z pass takes
+  // kz > kcut as zero without reading it)
   const int kyi = kwave(q / Nf, N), kzi = q % Nf;
   const bool live = kcut >= N / 2 || kyi * kyi + kzi * kzi <= kcut * (kcut + 1);
-  const bool wr = live || kzi > kcut;
+  const bool wr = live;
   C<T> u[N2], w[N2];
   if (live) {
 #pragma unroll
@@ -804,10 +828,10 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
 #pragma unroll
   for (int r = 0; r < N2; ++r) a[r] = Lf[posOut<N>(n1, r)];
   lineInvT<T, N, 0>(a, Lf, n1, true, wt);
-  // v_hat x-inverse: lines kz <= kcut outside the mask radius are zero and the
-  // next y pass reads none of their entries; the kz > kcut lines (zero too)
-  // are read as they are by the z pass
-  if (live || iz > A.kcut) {
+  // v_hat x-inverse: lines outside the mask radius are zero and are not
+  // written: the next y pass reads none of their entries, and the z pass takes
+  // kz > kcut as zero without reading it
+  if (live) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) st32(NLf, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], A.scale));
   }
@@ -921,8 +945,9 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs, in
   C<T>* L3 = lds + (3 * M + m) * (N + 1);
   C<T>* Lg = lds + (g * M + m) * (N + 1);
   // kcut < N/2: kn_hat is masked, so lines with ky^2 + kz^2 > kcut (kcut + 1)
-  // read no input (the previous y pass wrote none there) and give zero; of
-  // those, only the kz > kcut lines are written (the z pass reads them)
+  // read no input (the previous y pass wrote none there) and give zero, which
+  // is not written (the y pass skips those entries, the z pass takes kz > kcut
+  // as zero without reading it)
   const int kyl = kwave(iy, N);
   const bool live = kcut >= N / 2 || kyl * kyl + iz * iz <= kcut * (kcut + 1);
   C<T> a[N2];
@@ -954,7 +979,7 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs, in
     for (int r = 0; r < N2; ++r) a[r] = L0[posOut<N>(n1, r)];
   }
   lineInvT<T, N, 0>(a, L0, n1, g == 0, wt);
-  if (g == 0 && (live || iz > kcut)) {
+  if (g == 0 && live) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) st32(w, (uint32_t)(q + posIn<N>(n1, n2) * js), cscale<T>(a[n2], scale));
   }
