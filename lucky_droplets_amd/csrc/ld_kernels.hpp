@@ -638,6 +638,23 @@ __device__ __forceinline__ int xPairBlock() {
   const int x = pb % 8, k = pb / 8;
   return 2 * (8 * (k >> 1) + x) + (k & 1);
 }
+// The field-parallel x passes' lines q0 .. q0 + M - 1 ([ky][kz] order): true
+// if any holds a mode inside the dealiasing radius (ky^2 + kz^2 <=
+// kcut (kcut + 1)), or kcut >= N/2.  About 60 % of the blocks hold none at
+// kcut = N/3; where such a block's lines are zero and store nothing it exits
+// at entry instead of transforming zeros.
+template <int N, int M>
+__device__ __forceinline__ bool xBlockLive(int q0, int kcut) {
+  constexpr int Nf = N / 2 + 1;
+  if (kcut >= N / 2) return true;
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const int q = q0 + j, ky = kwave(q / Nf, N), kz = q % Nf;
+    any = any || ky * ky + kz * kz <= kcut * (kcut + 1);
+  }
+  return any;
+}
 // The kernels below give each of the three vector components its own thread
 // group (3 * M * N1 threads): every thread transforms one field, so a thread
 // holds N2 elements instead of 3 * N2, and the components meet in LDS for the
@@ -657,6 +674,7 @@ k_x_uw3(const C<T>* __restrict__ uk0, C<T>* __restrict__ out0, size_t fs, const 
   C<T>* __restrict__ out = rsel1(out0, rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
   __shared__ C<T> lds[3 * M * (N + 1)];
+  if (!xBlockLive<N, M>(xPairBlock() * M, kcut)) return;   // zero lines, nothing stored
   const int f = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
   const Twiddles<T, N> wt(n1, tw);         // line-FFT twiddles of n1, both directions
@@ -746,6 +764,8 @@ k_x_rhs3(RhsArgs<T> A0, const C<T>* __restrict__ tw, size_t rs) {
   const RhsArgs<T> A = A0.shifted(rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
   __shared__ C<T> lds[3 * M * (N + 1)];
+  // band: lines outside the mask radius need no RK update and store nothing
+  if (A.band && !xBlockLive<N, M>(xPairBlock() * M, A.kcut)) return;
   const int f = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
   const Twiddles<T, N> wt(n1, tw);         // line-FFT twiddles of n1, both directions
@@ -934,6 +954,7 @@ k_x_kn4(C<T>* w0, size_t fs, const C<T>* __restrict__ tw, T scale, size_t rs, in
   C<T>* w = rsel1(w0, rs);
   constexpr int N1 = Plan<N>::N1, N2 = Plan<N>::N2, Nf = N / 2 + 1, js = N * Nf, G = M * N1;
   __shared__ C<T> lds[4 * M * (N + 1)];
+  if (!xBlockLive<N, M>(xPairBlock() * M, kcut)) return;   // zero lines, nothing stored
   const int g = threadIdx.x / G, loc = threadIdx.x % G;
   const int m = loc % M, n1 = loc / M;
   const Twiddles<T, N> wt(n1, tw);         // line-FFT twiddles of n1, both directions
