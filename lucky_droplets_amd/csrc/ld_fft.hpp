@@ -107,8 +107,7 @@ template <int N, int Q = 0> __device__ __forceinline__ int xchg(int k2, int n1) 
 // line's LDS scratch (N elements).  Every thread of the block must call it
 // (it contains block barriers); threads with active == false only sync.
 // The inter-step twiddle W_N^(n1 k2) comes from twd(k2): a load of the table
-// tw[m] = exp(-2 pi i m / N) (lineFwd), or registers the caller filled
-// earlier (lineFwdW).
+// tw[m] = exp(-2 pi i m / N) (lineFwd), or a Twiddles source (below).
 template <typename T, int N, int Q, class TW>
 __device__ __forceinline__ void lineFwdT(C<T>* a, C<T>* L, int n1, bool active, TW&& twd) {
   constexpr int N1 = Plan<N, Q>::N1, N2 = Plan<N, Q>::N2, S = Plan<N, Q>::S;
@@ -138,11 +137,6 @@ __device__ __forceinline__ void lineFwdT(C<T>* a, C<T>* L, int n1, bool active, 
 template <typename T, int N, int Q = 0>
 __device__ __forceinline__ void lineFwd(C<T>* a, C<T>* L, int n1, bool active, const C<T>* __restrict__ tw) {
   lineFwdT<T, N, Q>(a, L, n1, active, [&](int k2) { return tw[n1 * k2]; });
-}
-// w[k2] = tw[n1 k2] already in registers (k2 = 1 .. N2 - 1)
-template <typename T, int N, int Q = 0>
-__device__ __forceinline__ void lineFwdW(C<T>* a, C<T>* L, int n1, bool active, const C<T>* w) {
-  lineFwdT<T, N, Q>(a, L, n1, active, [&](int k2) { return w[k2]; });
 }
 // Twiddles of thread n1 for both directions, loaded into registers.
 template <typename T, int N, int Q = 0>
@@ -193,10 +187,6 @@ __device__ __forceinline__ void lineInv(C<T>* a, C<T>* L, int n1, bool active, c
   } else {
     lineInvT<T, N, Q>(a, L, n1, active, [&](int k2) { return tw[n1 * k2]; });
   }
-}
-template <typename T, int N, int Q = 0>
-__device__ __forceinline__ void lineInvW(C<T>* a, C<T>* L, int n1, bool active, const C<T>* w) {
-  lineInvT<T, N, Q>(a, L, n1, active, [&](int k2) { return w[k2]; });
 }
 
 // Thread n1's twiddles for a kernel that runs several line transforms: held
