@@ -143,16 +143,18 @@ def time_large(cfg, uk_small, dt_small, steps, warmup):
     return ms, dt, blown
 
 
-def time_ensemble(cfg, nreal, steps, warmup, graph=True):
+def time_ensemble(cfg, nreal, steps, warmup, graph=True, autotune=True):
     """ms per ensemble step of `nreal` realizations of `cfg` stepped together
     (one launch per kernel for all realizations), each with its own seeded
-    initial field and droplets."""
+    initial field and droplets; with autotune the context picks its launch
+    geometries for this ensemble at creation (identical results)."""
     import numpy as np
     import torch
     from lucky_droplets_amd import ic
     from lucky_droplets_amd.dns import ForcedDNS
     dev = torch.device("cuda:0")
-    d = ForcedDNS(cfg, batch=nreal, graph=graph)
+    d = ForcedDNS(cfg, batch=nreal, graph=graph, autotune=autotune)
+    time_ensemble.last_tuning = d.tuning_all() if autotune else None
     uke = torch.empty((nreal,) + d.cshape(3), dtype=d.cdtype, device=dev)
     ne = torch.empty((nreal,) + d.rshape(), dtype=d.rdtype, device=dev)
     ce = torch.empty((nreal, cfg.n_particles, 7), dtype=torch.float64, device=dev)
@@ -507,15 +509,18 @@ def main():
         c3 = _preset("c3")
         nreal = 64
         log(f"[bench] ensemble: {nreal} x {c3.N}^3 FP32 + {c3.n_particles} droplets each")
-        mse, dte, blown_e = time_ensemble(c3, nreal, max(5, args.steps // 2), 2)
+        mse, dte, blown_e = time_ensemble(c3, nreal, max(5, args.steps // 2), 2, autotune=tuned)
+        etune = time_ensemble.last_tuning
         be = nreal * algorithmic_bytes(c3.N, 4, c3.n_particles, 40)
         out["ensemble"] = {"workload": f"{nreal} realizations of forced DNS {c3.N}^3 FP32 + {c3.n_particles} "
                                        "droplets, stepped together (one launch per kernel), HIP-graph replay",
                            "ms_per_ensemble_step": mse, "ensemble_steps_per_s": 1000.0 / mse,
                            "realization_steps_per_s": nreal * 1000.0 / mse, "dt": dte,
                            "bytes_per_ensemble_step": be, "achieved_GBs": be / (mse * 1e-3) / 1e9,
-                           "frac": be / (mse * 1e-3) / 1e9 / HBM_PEAK_GBS, "blowup": blown_e}
-        log(f"[bench] ensemble {mse:.3f} ms per ensemble step")
+                           "frac": be / (mse * 1e-3) / 1e9 / HBM_PEAK_GBS, "blowup": blown_e,
+                           "launch_tuning": etune[0] if etune else None}
+        log(f"[bench] ensemble {mse:.3f} ms per ensemble step"
+            + (f"; launch choices {etune[0]}" if etune else ""))
     if not args.no_cnn:
         ch = (3, 32, 32, 32, 3)
         log("[bench] learned subgrid CNN: 128^3, channels " + str(ch))
