@@ -331,7 +331,9 @@ def cpu_baseline(uk, n, coord, cfg, dt, c1_steps=10):
                       f"host: {model}, os.cpu_count()={ncpu}",
             "c1_64cubed_1e4_droplets": {"s_per_step_workers_1": c1_times[1], f"s_per_step_workers_{workers}":
                                         c1_times[workers], "steps_timed": c1_steps,
-                                        "est_200_steps_s_workers_1": 200 * c1_times[1]},
+                                        "est_200_steps_s_workers_1": 200 * c1_times[1],
+                                        f"s_for_{c1_steps}_steps_workers_1": c1_steps * c1_times[1],
+                                        f"s_for_{c1_steps}_steps_workers_{workers}": c1_steps * c1_times[workers]},
             "cpu_count": ncpu}
 
 
@@ -353,6 +355,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-subgrid-form comparison")
     ap.add_argument("--no-cnn", action="store_true", help="skip the learned-subgrid CNN timing")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default y-pass block shape")
+    ap.add_argument("--cpu-c1-steps", type=int, default=10,
+                    help="full oracle steps of the 64^3 workload in the CPU baseline (200 = the whole run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a "
@@ -533,8 +537,8 @@ def main():
         log(f"[bench] CNN forward {msf:.3f} ms, train step {mst:.3f} ms")
     out["blowup"] = bool(blown)
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle: one full step at the benchmark workload, 10 steps at 64^3)")
-        out["cpu_baseline"] = cpu_baseline(uk, n, coord, cfg, dt)
+        log(f"[bench] CPU baseline (oracle: one full step at the benchmark workload, {args.cpu_c1_steps} steps at 64^3)")
+        out["cpu_baseline"] = cpu_baseline(uk, n, coord, cfg, dt, c1_steps=args.cpu_c1_steps)
         log(f"[bench] CPU {out['cpu_baseline']['value']:.4f} steps/s")
     print(json.dumps(out), flush=True)
 
