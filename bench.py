@@ -455,6 +455,14 @@ def main():
         out["fp64_plain"] = {"ms_per_step": msp, "steps_per_s": 1000.0 / msp, "fp32_speedup": msp / ms32,
                              "form": "reference dataflow on the GPU: 74 separate 3-D transforms per step and one "
                                      "kernel per array statement (ld_advance_plain)"}
+        # the reference's own CPU-runnable case on the GPU, beside the CPU baseline's run of it
+        c1 = preset("c1")
+        uk1, n1, coord1, _, dt1 = make_inputs(c1)
+        ms_c1, _, _, _ = time_gpu(c1, uk1, n1, coord1, dt1, 200, 5)
+        log(f"[bench] C1 64^3 FP64 + {c1.n_particles} droplets: {ms_c1:.3f} ms/step, 200 steps in "
+            f"{0.2 * ms_c1:.2f} s")
+        out["c1_fp64"] = {"workload": f"forced DNS 64^3 FP64 + {c1.n_particles} droplets, 200 steps",
+                          "ms_per_step": ms_c1, "s_for_200_steps": 0.2 * ms_c1}
     if not args.no_fp64 and not args.no_accuracy:
         from lucky_droplets_amd.accuracy import compare_precisions
         nacc = cfg.steps
