@@ -656,6 +656,38 @@ def test_autotune_keeps_results(gpu):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+def test_step_queue_orders_give_the_same_bits(gpu):
+    """At N >= 128, where a stage's density chain is queued after the next
+    stage's x pass, one call of 3 steps, 3 calls of one step and HIP-graph
+    replay give the same state bit for bit."""
+    import torch
+    from lucky_droplets_amd import ic
+    from lucky_droplets_amd.dns import ForcedDNS
+    cfg = _cfg(128, "f32", 20000, nu=0.01)
+    rng = np.random.default_rng(41)
+    uk, n, u = ic.initial_flow(cfg, rng)
+    dt = ic.cfl_dt(cfg, u)
+    coord, _ = ic.initial_droplets(cfg, rng)
+    outs = []
+    for mode in ("one_call", "per_step", "graph"):
+        d = ForcedDNS(cfg, graph=(mode == "graph"))
+        t = (_t(uk, gpu, d.cdtype), _t(n, gpu, d.rdtype), _t(coord, gpu, torch.float64))
+        d.set_state(*t)
+        if mode == "one_call":
+            d.advance(dt, 3)
+        else:
+            for _ in range(3):
+                d.advance(dt, 1)
+        d.get_state(*t)
+        torch.cuda.synchronize()
+        outs.append([x.cpu().numpy() for x in t])
+        d.close()
+    assert np.all(np.isfinite(outs[0][0].view(np.float32)))
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
 @pytest.mark.parametrize("N,prec", [(64, "f32"), (32, "f64"), (256, "f32")])
 def test_every_launch_geometry_gives_the_same_bits(gpu, N, prec, monkeypatch):
     """The z-tile heights and x-pass line counts the autotuner chooses between
