@@ -124,8 +124,8 @@ __device__ __forceinline__ void store_blocked(bf16* __restrict__ Yblk, bf16* sT,
   __builtin_amdgcn_wave_barrier();
 }
 
-template <int CI, int CO, bool FLIP, bool RELU, bool MASK, int NT>
-__global__ void __launch_bounds__(256)
+template <int CI, int CO, bool FLIP, bool RELU, bool MASK, int NT, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
 k_conv2(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __restrict__ bias,
         const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N, int nseg,
         bf16* __restrict__ Yblk) {
@@ -134,8 +134,8 @@ k_conv2(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __r
   constexpr int KS = (27 * CI + 31) / 32;
   constexpr int WN = 27 * CO * CI;
   __shared__ __attribute__((aligned(16))) bf16 sW[WN];
-  __shared__ __attribute__((aligned(16))) bf16 sT[4][256];   // per-wavefront tile for the blocked copy
-  for (int e = threadIdx.x; e < WN / 8; e += 256)
+  __shared__ __attribute__((aligned(16))) bf16 sT[WPB][256];   // per-wavefront tile for the blocked copy
+  for (int e = threadIdx.x; e < WN / 8; e += 64 * WPB)
     reinterpret_cast<uint4*>(sW)[e] = reinterpret_cast<const uint4*>(W)[e];
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -144,7 +144,7 @@ k_conv2(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __r
   const int lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
   const int spl = N / (16 * NT);                  // segments per z-line
   const int msk = N - 1;
-  for (int seg = lb * 4 + wv; seg < nseg; seg += G * 4) {
+  for (int seg = lb * WPB + wv; seg < nseg; seg += G * WPB) {
     const int line = seg / spl, z0 = (seg % spl) * 16 * NT;
     const int x = line / N, y = line & msk;
     f32x4 acc[NT][MT];
@@ -241,16 +241,16 @@ __device__ __forceinline__ uint4 shift_down(const uint4& a, const uint4& prev) {
   return r;
 }
 
-template <int CO, bool FLIP, bool RELU, bool MASK, int NT>
-__global__ void __launch_bounds__(256)
+template <int CO, bool FLIP, bool RELU, bool MASK, int NT, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
 k_conv3(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __restrict__ bias,
         const bf16* __restrict__ mask, bf16* __restrict__ Yb, float* __restrict__ Yf, int N, int nseg,
         bf16* __restrict__ Yblk) {
   constexpr int CI = 32, MT = CO / 16;
   constexpr int WN = 27 * CO * CI;
   __shared__ __attribute__((aligned(16))) bf16 sW[WN];
-  __shared__ __attribute__((aligned(16))) bf16 sT[4][256];   // per-wavefront tile for the blocked copy
-  for (int e = threadIdx.x; e < WN / 8; e += 256)
+  __shared__ __attribute__((aligned(16))) bf16 sT[WPB][256];   // per-wavefront tile for the blocked copy
+  for (int e = threadIdx.x; e < WN / 8; e += 64 * WPB)
     reinterpret_cast<uint4*>(sW)[e] = reinterpret_cast<const uint4*>(W)[e];
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -259,7 +259,7 @@ k_conv3(const bf16* __restrict__ X, const bf16* __restrict__ W, const float* __r
   const int lb = (G % 8 == 0) ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
   const int spl = N / (16 * NT);
   const int msk = N - 1;
-  for (int seg = lb * 4 + wv; seg < nseg; seg += G * 4) {
+  for (int seg = lb * WPB + wv; seg < nseg; seg += G * WPB) {
     const int line = seg / spl, z0 = (seg % spl) * 16 * NT;
     const int x = line / N, y = line & msk;
     f32x4 acc[NT][MT];
